@@ -1,0 +1,344 @@
+"""bluecrunch_amd -- MI355X-native FFT bigfloat multiply behind bluecrunch's C API.
+
+The product is libbluecrunch.so (built in-tree by bluecrunch_amd/build.py): HIP
+kernels for gfx950 behind the C ABI of include/bluecrunch_gpu.h, plus the
+reference C API (bigfloat_mul, _fft_mul, fft.h) in C on top of it. This module
+is a thin ctypes view of that library for tests, the benchmark and Python
+callers. It never falls back to a CPU path: a missing or unloadable library is
+an error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbluecrunch.so")
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_sz = ctypes.c_size_t
+_vp = ctypes.c_void_p
+
+# Every extern "C" symbol include/*.h declares, grouped by header.
+EXPORTS = {
+    "bluecrunch_gpu.h": [
+        "bcg_init", "bcg_shutdown", "bcg_last_error", "bcg_stream", "bcg_sync", "bcg_malloc", "bcg_free",
+        "bcg_memcpy_h2d", "bcg_memcpy_d2h", "bcg_mul_dev", "bcg_mul_host", "bcg_mul_dev_bits", "bcg_basecase_dev",
+        "bcg_mul_batch_dev", "bcg_fft_forward_dev", "bcg_fft_inverse_dev", "bcg_fft_pointwise_dev",
+        "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
+        "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
+        "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_prof_enable", "bcg_prof_read",
+        "bcg_prof_reset",
+    ],
+    "fft.h": [
+        "twiddle_table", "twiddle_table_size", "fft_ensure_table", "table_select", "fft_free_table",
+        "fft_length", "fft_forward", "fft_inverse", "fft_pointwise", "int_to_fft", "fft_to_int",
+        "tft_forward", "tft_inverse", "baileys_forward",
+    ],
+    "mul.h": ["_basecase_mul", "_karatzuba_mul", "_fft_mul"],
+    "bigfloat.h": [
+        "bigfloat_zero", "bigfloat_iszero", "bigfloat_set", "bigfloat_free", "bigfloat_new", "bigfloat_copy",
+        "bigfloat_alloc", "bigfloat_realloc", "bigfloat_mul",
+    ],
+}
+
+PROF_KINDS = {
+    "fwd_pack": 0, "fwd": 1, "mid": 2, "inv": 3, "inv_unpack": 4, "carry": 5, "other": 6,
+}
+
+
+class MulInfo(ctypes.Structure):
+    _fields_ = [("bits", ctypes.c_int), ("passes", ctypes.c_int), ("points", _sz), ("margin", ctypes.c_double),
+                ("retries", ctypes.c_int), ("square", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Bigfloat(ctypes.Structure):
+    """include/bigfloat.h bigfloat_t layout."""
+    _fields_ = [("coef", _u32p), ("exp", ctypes.c_int64), ("len", _sz), ("sign", ctypes.c_int)]
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libbluecrunch.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GpuError(f"{LIB_PATH} is missing: run `python -m bluecrunch_amd.build` first")
+    L = ctypes.CDLL(LIB_PATH)
+    i = ctypes.c_int
+    L.bcg_init.argtypes = [i]
+    L.bcg_last_error.restype = ctypes.c_char_p
+    L.bcg_stream.restype = _vp
+    L.bcg_malloc.argtypes = [ctypes.POINTER(_vp), _sz]
+    L.bcg_free.argtypes = [_vp]
+    L.bcg_memcpy_h2d.argtypes = [_vp, _vp, _sz]
+    L.bcg_memcpy_d2h.argtypes = [_vp, _vp, _sz]
+    mul_args = [_vp, _sz, _vp, _sz, _vp, _sz]
+    L.bcg_mul_dev.argtypes = mul_args + [_vp, ctypes.POINTER(MulInfo)]
+    L.bcg_mul_dev_bits.argtypes = mul_args + [i, _vp, ctypes.POINTER(MulInfo)]
+    L.bcg_mul_host.argtypes = mul_args + [ctypes.POINTER(MulInfo)]
+    L.bcg_basecase_dev.argtypes = mul_args + [_vp]
+    L.bcg_basecase_host.argtypes = mul_args
+    L.bcg_mul_batch_dev.argtypes = [_vp, _vp, _vp] + [ctypes.POINTER(_sz)] * 6 + [_sz, _vp, _f64p]
+    L.bcg_fft_forward_dev.argtypes = [_vp, _sz, _vp]
+    L.bcg_fft_inverse_dev.argtypes = [_vp, _sz, _vp]
+    L.bcg_fft_pointwise_dev.argtypes = [_vp, _vp, _sz, _vp]
+    L.bcg_int_to_fft_dev.argtypes = [_vp, _sz, _vp, _sz, i, ctypes.POINTER(_sz), _vp]
+    L.bcg_fft_to_int_dev.argtypes = [_vp, _sz, _vp, _sz, i, _f64p, _vp]
+    L.bcg_fft_forward_host.argtypes = [_f64p, _sz]
+    L.bcg_fft_inverse_host.argtypes = [_f64p, _sz]
+    L.bcg_fft_pointwise_host.argtypes = [_f64p, _f64p, _sz]
+    L.bcg_int_to_fft_host.argtypes = [_f64p, _sz, _u32p, _sz, i, ctypes.POINTER(_sz)]
+    L.bcg_fft_to_int_host.argtypes = [_f64p, _sz, _u32p, _sz, i, _f64p]
+    L.bcg_tft_forward_host.argtypes = [_f64p, _sz, i]
+    L.bcg_tft_inverse_host.argtypes = [_f64p, _sz, i]
+    L.bcg_plan_query.argtypes = [_sz, _sz, ctypes.POINTER(i), ctypes.POINTER(_sz), ctypes.POINTER(i)]
+    L.bcg_prof_enable.argtypes = [i]
+    L.bcg_prof_read.argtypes = [i, ctypes.POINTER(ctypes.c_long), _f64p]
+    # reference C API
+    L.fft_length.argtypes = [_sz]; L.fft_length.restype = _sz
+    L.table_select.argtypes = [_sz]; L.table_select.restype = _sz
+    L.fft_ensure_table.argtypes = [_sz]; L.fft_ensure_table.restype = None
+    L.fft_forward.argtypes = [_f64p, _sz]; L.fft_forward.restype = None
+    L.fft_inverse.argtypes = [_f64p, _sz]; L.fft_inverse.restype = None
+    L.baileys_forward.argtypes = [_f64p, _sz]; L.baileys_forward.restype = None
+    L.fft_pointwise.argtypes = [_f64p, _f64p, _sz]; L.fft_pointwise.restype = None
+    L.int_to_fft.argtypes = [_f64p, _sz, _u32p, _sz, i]; L.int_to_fft.restype = _sz
+    L.fft_to_int.argtypes = [_f64p, _sz, _u32p, _sz, i]; L.fft_to_int.restype = None
+    L.tft_forward.argtypes = [_f64p, _sz, i]; L.tft_forward.restype = None
+    L.tft_inverse.argtypes = [_f64p, _sz, i]; L.tft_inverse.restype = None
+    for name in ("_basecase_mul", "_karatzuba_mul"):
+        getattr(L, name).argtypes = [_u32p, _sz, _u32p, _sz, _u32p, _sz]
+        getattr(L, name).restype = None
+    L._fft_mul.argtypes = [_u32p, _sz, _u32p, _sz, _u32p, _sz]
+    L._fft_mul.restype = i
+    L.bigfloat_mul.argtypes = [ctypes.POINTER(Bigfloat)] * 3 + [_sz]
+    L.bigfloat_mul.restype = None
+    L.bigfloat_free.argtypes = [ctypes.POINTER(Bigfloat)]
+    L.bigfloat_free.restype = None
+    _lib = L
+    return L
+
+
+def check(rc, what="bcg call"):
+    if rc != 0:
+        msg = lib().bcg_last_error()
+        raise GpuError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def exported_symbols():
+    """Names every header declares, all of which the library must export."""
+    return [s for names in EXPORTS.values() for s in names]
+
+
+def _u32(a):
+    return a.ctypes.data_as(_u32p)
+
+
+def _f64(a):
+    return a.ctypes.data_as(_f64p)
+
+
+# ------------------------------------------------------------ host-array API
+
+def mul(a, b, cl=None, square=None):
+    """Low cl words of a*b on the GPU (host arrays in, host array out)."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    sq = square if square is not None else (b is a)
+    b = a if sq else np.ascontiguousarray(b, dtype=np.uint32)
+    cl = a.size + b.size if cl is None else cl
+    c = np.zeros(max(cl, 1), dtype=np.uint32)
+    info = MulInfo()
+    check(lib().bcg_mul_host(c.ctypes.data, cl, a.ctypes.data, a.size, b.ctypes.data, b.size,
+                             ctypes.byref(info)), "bcg_mul_host")
+    return c[:cl], info.as_dict()
+
+
+def basecase(a, b, cl=None):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    cl = a.size + b.size if cl is None else cl
+    c = np.zeros(max(cl, 1), dtype=np.uint32)
+    check(lib().bcg_basecase_host(c.ctypes.data, cl, a.ctypes.data, a.size, b.ctypes.data, b.size),
+          "bcg_basecase_host")
+    return c[:cl]
+
+
+def plan(al, bl):
+    bits, pts, passes = ctypes.c_int(), _sz(), ctypes.c_int()
+    check(lib().bcg_plan_query(al, bl, ctypes.byref(bits), ctypes.byref(pts), ctypes.byref(passes)), "plan")
+    return {"bits": bits.value, "points": pts.value, "passes": passes.value}
+
+
+# Reference C API (include/fft.h, include/mul.h, include/bigfloat.h)
+
+def fft_length(n):
+    return lib().fft_length(n)
+
+
+def fft_forward(z):
+    assert z.dtype == np.complex128 and z.flags.c_contiguous
+    lib().fft_forward(_f64(z.view(np.float64)), z.size)
+    return z
+
+
+def fft_inverse(z):
+    assert z.dtype == np.complex128 and z.flags.c_contiguous
+    lib().fft_inverse(_f64(z.view(np.float64)), z.size)
+    return z
+
+
+def baileys_forward(z):
+    lib().baileys_forward(_f64(z.view(np.float64)), z.size)
+    return z
+
+
+def fft_pointwise(a, b):
+    lib().fft_pointwise(_f64(a.view(np.float64)), _f64(b.view(np.float64)), a.size)
+    return a
+
+
+def int_to_fft(length, words, bits):
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    pts = (32 * words.size + bits - 1) // bits if words.size else 0
+    z = np.zeros(max(length, pts, 1), dtype=np.complex128)
+    n = lib().int_to_fft(_f64(z.view(np.float64)), length, _u32(words), words.size, bits)
+    return z, n
+
+
+def fft_to_int(z, length, nwords, bits):
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    out = np.zeros(max(nwords, 1), dtype=np.uint32)
+    lib().fft_to_int(_f64(z.view(np.float64)), length, _u32(out), nwords, bits)
+    return out[:nwords]
+
+
+def fft_to_int_margin(z, length, nwords, bits):
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    out = np.zeros(max(nwords, 1), dtype=np.uint32)
+    m = ctypes.c_double()
+    check(lib().bcg_fft_to_int_host(_f64(z.view(np.float64)), length, _u32(out), nwords, bits, ctypes.byref(m)),
+          "fft_to_int")
+    return out[:nwords], m.value
+
+
+def fft_mul(a, b, cl=None):
+    """The reference _fft_mul entry point; returns (words, bits)."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    cl = a.size + b.size if cl is None else cl
+    c = np.zeros(max(cl, 1), dtype=np.uint32)
+    bits = lib()._fft_mul(_u32(c), cl, _u32(a), a.size, _u32(b), b.size)
+    return c[:cl], bits
+
+
+def bigfloat_mul(a, b, p=0, inplace=None):
+    """Reference bigfloat_mul on (words, exp, sign) triples; `a is b` is squaring.
+    inplace="a" makes the target the same bigfloat as a (test/test_regression.c)."""
+    L = lib()
+    libc = ctypes.CDLL(None)
+    libc.malloc.restype = _vp
+    libc.malloc.argtypes = [_sz]
+
+    def mk(t):
+        w, e, s = t
+        w = np.ascontiguousarray(w, dtype=np.uint32)
+        bf = Bigfloat()
+        if w.size:
+            p_ = libc.malloc(4 * w.size)
+            ctypes.memmove(p_, w.ctypes.data, 4 * w.size)
+            bf.coef = ctypes.cast(p_, _u32p)
+        else:
+            bf.coef = ctypes.cast(None, _u32p)
+        bf.exp, bf.len, bf.sign = e, w.size, s
+        return bf
+
+    A = mk(a)
+    B = A if b is a else mk(b)
+    if inplace == "a":
+        T = A
+    else:
+        T = Bigfloat()
+        T.coef = ctypes.cast(None, _u32p)
+        T.len, T.exp, T.sign = 0, 0, 1
+    L.bigfloat_mul(ctypes.byref(T), ctypes.byref(A), ctypes.byref(B), p)
+    words = np.ctypeslib.as_array(T.coef, shape=(T.len,)).copy() if T.len else np.zeros(0, np.uint32)
+    res = (words, T.exp, T.sign)
+    for bf in {id(x): x for x in (A, B, T)}.values():
+        L.bigfloat_free(ctypes.byref(bf))
+    return res
+
+
+# ---------------------------------------------------------- device buffers
+
+class DeviceWords:
+    """Device array of uint32 words allocated through the library."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        p = _vp()
+        check(lib().bcg_malloc(ctypes.byref(p), max(4 * self.n, 4)), "bcg_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.uint32)
+        d = cls(arr.size)
+        if arr.size:
+            check(lib().bcg_memcpy_h2d(d.ptr, arr.ctypes.data, 4 * arr.size), "h2d")
+        return d
+
+    def to_host(self, n=None):
+        n = self.n if n is None else n
+        out = np.zeros(max(n, 1), dtype=np.uint32)
+        if n:
+            check(lib().bcg_memcpy_d2h(out.ctypes.data, self.ptr, 4 * n), "d2h")
+        return out[:n]
+
+    def free(self):
+        if self.ptr:
+            lib().bcg_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def mul_dev(c, cl, a, al, b, bl, bits=0, stream=None):
+    """Device-pointer multiply (ints are raw device addresses)."""
+    info = MulInfo()
+    check(lib().bcg_mul_dev_bits(c, cl, a, al, b, bl, bits, stream, ctypes.byref(info)), "bcg_mul_dev")
+    return info.as_dict()
+
+
+def prof_enable(on=True):
+    check(lib().bcg_prof_enable(1 if on else 0), "prof_enable")
+
+
+def prof_read(kind):
+    n, ms = ctypes.c_long(), ctypes.c_double()
+    check(lib().bcg_prof_read(PROF_KINDS[kind] if isinstance(kind, str) else kind, ctypes.byref(n),
+                              ctypes.byref(ms)), "prof_read")
+    return n.value, ms.value
+
+
+def prof_reset():
+    check(lib().bcg_prof_reset(), "prof_reset")
+
+
+def sync():
+    check(lib().bcg_sync(), "sync")

@@ -1,0 +1,1170 @@
+// Host side of the HIP engine: plans, root-of-unity tables, launch sequences
+// and the extern "C" ABI declared in include/bluecrunch_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bluecrunch_gpu.h"
+#include "kernels.h"
+
+using namespace bcg;
+
+namespace {
+
+// ------------------------------------------------------------------ state
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct RootsDev {
+  cd *lo = nullptr;
+  cd *hi = nullptr;
+  int sh = 0;
+  unsigned long long four_n = 0;
+};
+
+struct Pending {
+  int kind;
+  hipEvent_t a, b;
+};
+
+struct Engine {
+  bool ready = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<unsigned long long, RootsDev> roots;  // keyed by N
+  std::map<int, cd *> ltab;                      // e^{2 pi i t/L}, keyed by L
+  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf;
+  bool prof = false;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> free_events;
+  long prof_n[BCG_PROF_KINDS] = {0};
+  double prof_ms[BCG_PROF_KINDS] = {0};
+};
+
+Engine g;
+std::recursive_mutex g_lock;
+thread_local std::string g_err;
+
+int fail(int code, const char *what, hipError_t e = hipSuccess) {
+  char buf[512];
+  if (e != hipSuccess)
+    snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  else
+    snprintf(buf, sizeof buf, "%s", what);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                      \
+  do {                                                 \
+    hipError_t e_ = (x);                               \
+    if (e_ != hipSuccess) return fail(BCG_E_HIP, #x, e_); \
+  } while (0)
+
+int ensure(DevBuf &b, size_t bytes) {
+  if (b.bytes >= bytes) return 0;
+  if (b.p) {
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes < 256 ? 256 : bytes;
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(BCG_E_NOMEM, "device allocation failed", e);
+  }
+  b.bytes = want;
+  return 0;
+}
+
+int init_locked(int device) {
+  if (g.ready) return 0;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) return fail(BCG_E_NODEV, "no HIP device", e);
+  if (device < 0 || device >= n) return fail(BCG_E_ARG, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  g.device = device;
+  g.ready = true;
+  if (int rc = ensure(g.margin, 256)) return rc;
+  return 0;
+}
+
+#define ENTER()                                   \
+  std::lock_guard<std::recursive_mutex> lk_(g_lock); \
+  do {                                            \
+    if (int rc_ = init_locked(0)) return rc_;     \
+    (void)hipSetDevice(g.device);                 \
+  } while (0)
+
+hipStream_t pick(void *s) { return s ? (hipStream_t)s : g.stream; }
+
+// ----------------------------------------------------------------- timing
+
+hipEvent_t get_event() {
+  if (!g.free_events.empty()) {
+    hipEvent_t e = g.free_events.back();
+    g.free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+struct ProfScope {
+  int kind;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  ProfScope(int k, hipStream_t st) : kind(k), s(st) {
+    if (g.prof) {
+      a = get_event();
+      (void)hipEventRecord(a, s);
+    }
+  }
+  ~ProfScope() {
+    if (g.prof && a) {
+      hipEvent_t b = get_event();
+      (void)hipEventRecord(b, s);
+      g.pending.push_back({kind, a, b});
+    }
+  }
+};
+
+void prof_collect() {
+  for (auto &p : g.pending) {
+    (void)hipEventSynchronize(p.b);
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      g.prof_n[p.kind]++;
+      g.prof_ms[p.kind] += ms;
+    }
+    g.free_events.push_back(p.a);
+    g.free_events.push_back(p.b);
+  }
+  g.pending.clear();
+}
+
+// ----------------------------------------------------------------- tables
+
+// cos/sin of 2 pi num/den in long double (64-bit significand), reduced to the
+// first octant so the long double argument stays small and exact-ish.
+void root_ld(unsigned long long num, unsigned long long den, double *re, double *im) {
+  // e^{2 pi i num / den}; reduce num/den to [0, 1/8] with symmetries.
+  num %= den;
+  // work with x = num/den in [0,1)
+  long double two_pi = 6.283185307179586476925286766559005768L;
+  unsigned long long n8 = num * 8;  // compare against den multiples
+  int oct = (int)(n8 / den);        // 0..7
+  // angle = 2 pi num/den = oct * pi/4 + r, r in [0, pi/4)
+  long double r = two_pi * (long double)(n8 - (unsigned long long)oct * den) / (8.0L * (long double)den);
+  long double c = cosl(r), s = sinl(r);
+  long double h = 0.70710678118654752440084436210484903928L;
+  long double cr, ci;
+  switch (oct) {
+    case 0: cr = c; ci = s; break;
+    case 1: cr = h * (c - s); ci = h * (c + s); break;
+    case 2: cr = -s; ci = c; break;
+    case 3: cr = -h * (c + s); ci = h * (c - s); break;
+    case 4: cr = -c; ci = -s; break;
+    case 5: cr = -h * (c - s); ci = -h * (c + s); break;
+    case 6: cr = s; ci = -c; break;
+    default: cr = h * (c + s); ci = -h * (c - s); break;
+  }
+  *re = (double)cr;
+  *im = (double)ci;
+}
+
+int upload(cd **dst, const std::vector<double> &host) {
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, host.size() * sizeof(double));
+  if (e != hipSuccess) return fail(BCG_E_NOMEM, "table allocation failed", e);
+  HIPCHK(hipMemcpy(p, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+  *dst = (cd *)p;
+  return 0;
+}
+
+int get_roots(unsigned long long N, Roots *out) {
+  auto it = g.roots.find(N);
+  if (it == g.roots.end()) {
+    RootsDev r;
+    r.four_n = 4 * N;
+    int lg = 0;
+    while ((1ull << lg) < r.four_n) lg++;
+    r.sh = (lg + 1) / 2;
+    unsigned long long nlo = 1ull << r.sh;
+    unsigned long long nhi = (r.four_n + nlo - 1) >> r.sh;
+    std::vector<double> lo(2 * nlo), hi(2 * nhi);
+    for (unsigned long long u = 0; u < nlo; u++) root_ld(u, r.four_n, &lo[2 * u], &lo[2 * u + 1]);
+    for (unsigned long long u = 0; u < nhi; u++) root_ld(u << r.sh, r.four_n, &hi[2 * u], &hi[2 * u + 1]);
+    if (int rc = upload(&r.lo, lo)) return rc;
+    if (int rc = upload(&r.hi, hi)) return rc;
+    it = g.roots.emplace(N, r).first;
+  }
+  out->lo = it->second.lo;
+  out->hi = it->second.hi;
+  out->sh = it->second.sh;
+  out->four_n = it->second.four_n;
+  return 0;
+}
+
+int get_ltab(int L, const cd **out) {
+  auto it = g.ltab.find(L);
+  if (it == g.ltab.end()) {
+    int n = L / 2 > 0 ? L / 2 : 1;
+    std::vector<double> t(2 * n);
+    for (int i = 0; i < n; i++) root_ld((unsigned long long)i, (unsigned long long)L, &t[2 * i], &t[2 * i + 1]);
+    cd *p = nullptr;
+    if (int rc = upload(&p, t)) return rc;
+    it = g.ltab.emplace(L, p).first;
+  }
+  *out = it->second;
+  return 0;
+}
+
+// ------------------------------------------------------------------ plans
+
+bool is_three(unsigned long long n) { return n % 3 == 0; }
+
+// Smallest 2^k or 3*2^k >= n (fft/fft.c:22-41 length classes), at least 64.
+unsigned long long class_ceil(unsigned long long n) {
+  unsigned long long b = 32;
+  for (;;) {
+    if (2 * b >= n) return 2 * b;
+    if (3 * b >= n) return 3 * b;
+    b *= 2;
+  }
+}
+
+int ilog2u(unsigned long long v) {
+  int l = 0;
+  while ((1ull << (l + 1)) <= v) l++;
+  return l;
+}
+
+// Largest digit size whose worst rounding error stays well below 1/4 at
+// transform length N (unsigned digits, right-angle convolution).
+int bmax_for(unsigned long long N) {
+  double lg = std::log2((double)N);
+  double b = (50.5 - lg - std::log2(lg)) / 2.0;
+  int bi = (int)std::floor(b);
+  if (bi > 20) bi = 20;
+  if (bi < 4) bi = 4;
+  return bi;
+}
+
+struct Plan {
+  unsigned long long N = 0;
+  int bits = 0;
+  int P = 0;
+  int L[3] = {0, 0, 0};
+  int C[3] = {1, 1, 1};
+  unsigned long long M[3] = {1, 1, 1};
+  unsigned long long dA = 0, dB = 0;
+};
+
+unsigned long long digits_of(unsigned long long words, int bits) {
+  return (32ull * words + (unsigned long long)bits - 1) / (unsigned long long)bits;
+}
+
+int tile_cols(int L, unsigned long long M) {
+  int c = 8192 / L;
+  if (is_three((unsigned long long)L)) c = 6144 / L;
+  if (c > 16) c = 16;
+  if (c < 1) c = 1;
+  while ((unsigned long long)c > M) c >>= 1;
+  return c;
+}
+
+// Split N into passes: innermost contiguous pass up to 8192 (6144 for 3*2^k),
+// outer passes >= 64 rows.
+bool factor(unsigned long long N, Plan &p) {
+  const unsigned long long MAXIN = is_three(N) ? 6144 : 8192;
+  if (N <= MAXIN) {
+    p.P = 1;
+    p.L[0] = (int)N;
+    return true;
+  }
+  unsigned long long p2 = is_three(N) ? N / 3 : N;  // power of two part
+  unsigned long long inner = MAXIN;
+  if (N / inner < 64) inner = N / 64;  // keep the outer pass at least 64 long
+  unsigned long long outer = N / inner;
+  (void)p2;
+  if (outer <= 8192) {
+    p.P = 2;
+    p.L[0] = (int)outer;
+    p.L[1] = (int)inner;
+  } else {
+    unsigned long long rest = outer;  // power of two, split into two passes
+    int lg = ilog2u(rest);
+    unsigned long long l1 = 1ull << ((lg + 1) / 2);
+    unsigned long long l2 = rest / l1;
+    if (l1 > 8192 || l2 > 8192 || l2 < 64) return false;
+    p.P = 3;
+    p.L[0] = (int)l1;
+    p.L[1] = (int)l2;
+    p.L[2] = (int)inner;
+  }
+  unsigned long long m = N;
+  for (int i = 0; i < p.P; i++) {
+    m /= (unsigned long long)p.L[i];
+    p.M[i] = m;
+    p.C[i] = (i == p.P - 1) ? 1 : tile_cols(p.L[i], m);
+  }
+  return true;
+}
+
+int make_plan(unsigned long long AL, unsigned long long BL, int force_bits, Plan &best) {
+  best = Plan();
+  for (int b = 20; b >= 4; b--) {
+    if (force_bits > 0 && b != force_bits) continue;
+    unsigned long long dA = digits_of(AL, b), dB = digits_of(BL, b);
+    unsigned long long need = dA + dB - 1;
+    unsigned long long N = class_ceil((need + 1) / 2);
+    if (force_bits <= 0 && b > bmax_for(N)) continue;
+    if (best.N == 0 || N < best.N || (N == best.N && b < best.bits)) {
+      Plan p;
+      p.N = N;
+      p.bits = b;
+      p.dA = dA;
+      p.dB = dB;
+      if (!factor(N, p)) continue;
+      best = p;
+    }
+  }
+  if (best.N == 0) return fail(BCG_E_ARG, "no transform plan for this size");
+  return 0;
+}
+
+// --------------------------------------------------------------- launching
+
+#define LAUNCH(kern, grid, block, stream, ...) \
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, __VA_ARGS__)
+
+template <int L, int C>
+int launch_fwd(bool pack, int grid, hipStream_t s, const PassArgs &a) {
+  if (pack)
+    LAUNCH((k_fwd<L, C, true>), grid, (Tile<L, C>::T), s, a);
+  else
+    LAUNCH((k_fwd<L, C, false>), grid, (Tile<L, C>::T), s, a);
+  return 0;
+}
+
+template <int L, int C>
+int launch_inv(bool unpack, int grid, hipStream_t s, const PassArgs &a) {
+  if (unpack)
+    LAUNCH((k_inv<L, C, true>), grid, (Tile<L, C>::T), s, a);
+  else
+    LAUNCH((k_inv<L, C, false>), grid, (Tile<L, C>::T), s, a);
+  return 0;
+}
+
+template <int L>
+int launch_mid(bool sq, bool pack, int grid, hipStream_t s, const PassArgs &a) {
+  constexpr int T = Tile<L, 1>::T;
+  if (pack) {
+    if (sq)
+      LAUNCH((k_mid<L, true, true, true>), grid, T, s, a);
+    else
+      LAUNCH((k_mid<L, false, true, true>), grid, T, s, a);
+  } else {
+    if (sq)
+      LAUNCH((k_mid<L, true, false, false>), grid, T, s, a);
+    else
+      LAUNCH((k_mid<L, false, false, false>), grid, T, s, a);
+  }
+  return 0;
+}
+
+// Dispatch over the instantiated (L, C) tiles.
+#define FOR_C(L, X)           \
+  case 1: return X<L, 1>;     \
+  case 2: return X<L, 2>;     \
+  case 4: return X<L, 4>;     \
+  case 8: return X<L, 8>;     \
+  case 16: return X<L, 16>;
+
+typedef int (*fwd_fn)(bool, int, hipStream_t, const PassArgs &);
+
+template <int L>
+fwd_fn pick_fwd_c(int C) {
+  switch (C) {
+    case 1: return &launch_fwd<L, 1>;
+    case 2: return (L * 2 <= 8192) ? &launch_fwd<L, (L * 2 <= 8192 ? 2 : 1)> : nullptr;
+    case 4: return (L * 4 <= 8192) ? &launch_fwd<L, (L * 4 <= 8192 ? 4 : 1)> : nullptr;
+    case 8: return (L * 8 <= 8192) ? &launch_fwd<L, (L * 8 <= 8192 ? 8 : 1)> : nullptr;
+    case 16: return (L * 16 <= 8192) ? &launch_fwd<L, (L * 16 <= 8192 ? 16 : 1)> : nullptr;
+  }
+  return nullptr;
+}
+
+template <int L>
+fwd_fn pick_inv_c(int C) {
+  switch (C) {
+    case 1: return &launch_inv<L, 1>;
+    case 2: return (L * 2 <= 8192) ? &launch_inv<L, (L * 2 <= 8192 ? 2 : 1)> : nullptr;
+    case 4: return (L * 4 <= 8192) ? &launch_inv<L, (L * 4 <= 8192 ? 4 : 1)> : nullptr;
+    case 8: return (L * 8 <= 8192) ? &launch_inv<L, (L * 8 <= 8192 ? 8 : 1)> : nullptr;
+    case 16: return (L * 16 <= 8192) ? &launch_inv<L, (L * 16 <= 8192 ? 16 : 1)> : nullptr;
+  }
+  return nullptr;
+}
+
+fwd_fn pick_fwd(int L, int C) {
+  switch (L) {
+    case 64: return pick_fwd_c<64>(C);
+    case 128: return pick_fwd_c<128>(C);
+    case 256: return pick_fwd_c<256>(C);
+    case 512: return pick_fwd_c<512>(C);
+    case 1024: return pick_fwd_c<1024>(C);
+    case 2048: return pick_fwd_c<2048>(C);
+    case 4096: return pick_fwd_c<4096>(C);
+    case 8192: return pick_fwd_c<8192>(C);
+    case 96: return pick_fwd_c<96>(C);
+    case 192: return pick_fwd_c<192>(C);
+    case 384: return pick_fwd_c<384>(C);
+    case 768: return pick_fwd_c<768>(C);
+    case 1536: return pick_fwd_c<1536>(C);
+    case 3072: return pick_fwd_c<3072>(C);
+    case 6144: return pick_fwd_c<6144>(C);
+  }
+  return nullptr;
+}
+
+fwd_fn pick_inv(int L, int C) {
+  switch (L) {
+    case 64: return pick_inv_c<64>(C);
+    case 128: return pick_inv_c<128>(C);
+    case 256: return pick_inv_c<256>(C);
+    case 512: return pick_inv_c<512>(C);
+    case 1024: return pick_inv_c<1024>(C);
+    case 2048: return pick_inv_c<2048>(C);
+    case 4096: return pick_inv_c<4096>(C);
+    case 8192: return pick_inv_c<8192>(C);
+    case 96: return pick_inv_c<96>(C);
+    case 192: return pick_inv_c<192>(C);
+    case 384: return pick_inv_c<384>(C);
+    case 768: return pick_inv_c<768>(C);
+    case 1536: return pick_inv_c<1536>(C);
+    case 3072: return pick_inv_c<3072>(C);
+    case 6144: return pick_inv_c<6144>(C);
+  }
+  return nullptr;
+}
+
+typedef int (*mid_fn)(bool, bool, int, hipStream_t, const PassArgs &);
+
+mid_fn pick_mid(int L) {
+  switch (L) {
+    case 64: return &launch_mid<64>;
+    case 128: return &launch_mid<128>;
+    case 256: return &launch_mid<256>;
+    case 512: return &launch_mid<512>;
+    case 1024: return &launch_mid<1024>;
+    case 2048: return &launch_mid<2048>;
+    case 4096: return &launch_mid<4096>;
+    case 8192: return &launch_mid<8192>;
+    case 96: return &launch_mid<96>;
+    case 192: return &launch_mid<192>;
+    case 384: return &launch_mid<384>;
+    case 768: return &launch_mid<768>;
+    case 1536: return &launch_mid<1536>;
+    case 3072: return &launch_mid<3072>;
+    case 6144: return &launch_mid<6144>;
+  }
+  return nullptr;
+}
+
+int coef_bits(const Plan &p) {
+  unsigned long long m = p.dA < p.dB ? p.dA : p.dB;
+  int cb = 2 * p.bits + ilog2u(m + 1) + 2;
+  return cb > 63 ? 63 : cb;
+}
+
+// Fill the common pass arguments for pass i of plan p.
+int pass_args(const Plan &p, int i, PassArgs &a) {
+  a.N = p.N;
+  a.M = p.M[i];
+  a.S = (unsigned long long)p.L[i] * p.M[i];
+  a.tw_mul = 4ull * p.N / a.S;
+  if (int rc = get_ltab(p.L[i], &a.tl)) return rc;
+  if (int rc = get_roots(p.N, &a.roots)) return rc;
+  a.bits = p.bits;
+  a.inv_n = 1.0 / (double)p.N;
+  a.ntiles = (int)(p.N / ((unsigned long long)p.L[i] * p.C[i]));
+  return 0;
+}
+
+// Carry resolve: coefficients -> CL words.
+int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbits, uint32_t *out,
+              unsigned long long CL, hipStream_t s) {
+  if (CL == 0) return 0;
+  const unsigned long long wpb = (unsigned long long)CT * CWPT;
+  unsigned long long nblk = (CL + wpb - 1) / wpb;
+  if (int rc = ensure(g.blk, nblk)) return rc;
+  CarryArgs c;
+  c.coef = coef;
+  c.ncoef = ncoef;
+  c.bits = bits;
+  c.cbits = cbits;
+  c.out = out;
+  c.CL = CL;
+  c.blk_fn = (unsigned char *)g.blk.p;
+  c.nblk = nblk;
+  {
+    ProfScope ps(BCG_K_CARRY, s);
+    LAUNCH(k_carry_local, (int)nblk, CT, s, c);
+    LAUNCH(k_carry_fix, 1, 1024, s, c);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int zero_margin(hipStream_t s) {
+  LAUNCH(k_zero_u32, 1, 64, s, (unsigned int *)g.margin.p, 1);
+  return 0;
+}
+
+int read_margin(hipStream_t s, double *m) {
+  unsigned int bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, g.margin.p, sizeof bits, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  float f;
+  memcpy(&f, &bits, sizeof f);
+  *m = (double)f;
+  return 0;
+}
+
+// Run the transforms of plan p up to the rounded coefficients (in bufB).
+int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, const uint32_t *B,
+                   unsigned long long BL, bool square, hipStream_t s) {
+  const size_t arr = (size_t)p.N * sizeof(cd);
+  if (int rc = ensure(g.bufA, arr)) return rc;
+  if (int rc = ensure(g.bufB, arr)) return rc;  // B spectrum, then coefficients (2N x 8 bytes)
+  cd *da = (cd *)g.bufA.p;
+  cd *db = (cd *)g.bufB.p;
+  long long *coef = (long long *)g.bufB.p;
+  if (int rc = zero_margin(s)) return rc;
+  PassArgs a;
+  memset(&a, 0, sizeof a);
+  a.wa = A;
+  a.al = AL;
+  a.wb = B;
+  a.bl = BL;
+  a.margin = (unsigned int *)g.margin.p;
+  if (p.P == 1) {
+    if (int rc = pass_args(p, 0, a)) return rc;
+    a.data = da;
+    a.data_b = db;
+    a.coef = coef;
+    mid_fn f = pick_mid(p.L[0]);
+    if (!f) return fail(BCG_E_ARG, "no kernel for this length");
+    ProfScope ps(BCG_K_MID, s);
+    f(square, true, 1, s, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  // forward outer passes for A (and B)
+  for (int op = 0; op < (square ? 1 : 2); op++) {
+    cd *buf = op == 0 ? da : db;
+    for (int i = 0; i < p.P - 1; i++) {
+      PassArgs b = a;
+      if (int rc = pass_args(p, i, b)) return rc;
+      b.data = buf;
+      b.wa = op == 0 ? A : B;
+      b.al = op == 0 ? AL : BL;
+      fwd_fn f = pick_fwd(p.L[i], p.C[i]);
+      if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
+      ProfScope ps(i == 0 ? BCG_K_FWD_PACK : BCG_K_FWD, s);
+      f(i == 0, b.ntiles, s, b);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  // middle: last forward pass + pointwise + first inverse pass, in place in A
+  {
+    PassArgs b = a;
+    int i = p.P - 1;
+    if (int rc = pass_args(p, i, b)) return rc;
+    b.data = da;
+    b.data_b = db;
+    mid_fn f = pick_mid(p.L[i]);
+    if (!f) return fail(BCG_E_ARG, "no middle kernel for this length");
+    ProfScope ps(BCG_K_MID, s);
+    f(square, false, (int)(p.N / (unsigned long long)p.L[i]), s, b);
+    HIPCHK(hipGetLastError());
+  }
+  // inverse outer passes, the outermost one rounding into coefficients
+  for (int i = p.P - 2; i >= 0; i--) {
+    PassArgs b = a;
+    if (int rc = pass_args(p, i, b)) return rc;
+    b.data = da;
+    b.coef = coef;
+    fwd_fn f = pick_inv(p.L[i], p.C[i]);
+    if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
+    ProfScope ps(i == 0 ? BCG_K_INV_UNPACK : BCG_K_INV, s);
+    f(i == 0, b.ntiles, s, b);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+__global__ void k_basecase(const uint32_t *A, unsigned long long AL, const uint32_t *B, unsigned long long BL,
+                           uint32_t *C, unsigned long long CL) {
+  // column sums of 32x32-bit products split into low and high halves
+  extern __shared__ unsigned long long colsum[];
+  const unsigned long long W = AL + BL;
+  for (unsigned long long w = threadIdx.x; w < W; w += blockDim.x) {
+    unsigned long long lo = 0, hi = 0;
+    unsigned long long i0 = w >= BL ? w - BL + 1 : 0;
+    for (unsigned long long i = i0; i < AL && i <= w; i++) {
+      unsigned long long pr = (unsigned long long)A[i] * B[w - i];
+      lo += pr & 0xffffffffull;
+      hi += pr >> 32;
+    }
+    colsum[2 * w] = lo;
+    colsum[2 * w + 1] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long carry = 0;  // < 2^40
+    for (unsigned long long w = 0; w < CL; w++) {
+      unsigned long long v = carry;
+      if (w < W) v += colsum[2 * w];
+      if (w >= 1 && w - 1 < W) v += colsum[2 * (w - 1) + 1];
+      C[w] = (uint32_t)v;
+      carry = v >> 32;
+    }
+  }
+}
+
+int basecase_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                 hipStream_t s) {
+  if (AL + BL > 4000) return fail(BCG_E_ARG, "basecase limited to AL + BL <= 4000 words");
+  size_t sh = (AL + BL + 1) * 2 * sizeof(unsigned long long);
+  // write into a scratch so C may alias A or B
+  if (int rc = ensure(g.zbuf, CL * sizeof(uint32_t) + 64)) return rc;
+  {
+    ProfScope ps(BCG_K_OTHER, s);
+    hipLaunchKernelGGL(k_basecase, dim3(1), dim3(256), sh, s, A, (unsigned long long)AL, B,
+                       (unsigned long long)BL, (uint32_t *)g.zbuf.p, (unsigned long long)CL);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(C, g.zbuf.p, CL * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+int fill_zero(uint32_t *C, size_t CL, hipStream_t s) {
+  if (!CL) return 0;
+  int grid = (int)((CL + 255) / 256);
+  if (grid > 4096) grid = 4096;
+  LAUNCH(k_fill_u32, grid, 256, s, C, (unsigned long long)CL, 0u);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                   int force_bits, hipStream_t s, bcg_mul_info *info) {
+  bcg_mul_info inf;
+  memset(&inf, 0, sizeof inf);
+  bool square = (A == B && AL == BL);
+  inf.square = square ? 1 : 0;
+  if (AL == 0 || BL == 0 || CL == 0) {
+    if (info) *info = inf;
+    return fill_zero(C, CL, s);
+  }
+  if (AL + BL < 100 && force_bits <= 0) {
+    if (info) *info = inf;
+    return basecase_dev(C, CL, A, AL, B, BL, s);
+  }
+  Plan p;
+  if (int rc = make_plan(AL, BL, force_bits, p)) return rc;
+  bool alias = (C == A || C == B);
+  for (;;) {
+    if (int rc = run_transforms(p, A, AL, B, BL, square, s)) return rc;
+    double m = 0;
+    if (alias) {
+      // the carry pass overwrites the inputs: check the margin first
+      if (int rc = read_margin(s, &m)) return rc;
+    }
+    if (!alias) {
+      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s)) return rc;
+      if (int rc = read_margin(s, &m)) return rc;
+    }
+    inf.bits = p.bits;
+    inf.passes = p.P;
+    inf.points = (size_t)p.N;
+    inf.margin = m;
+    if (m >= 0.25) {
+      if (p.bits <= 4) {
+        if (info) *info = inf;
+        return fail(BCG_E_MARGIN, "rounding margin >= 0.25 at the smallest digit size");
+      }
+      Plan q;
+      if (int rc = make_plan(AL, BL, p.bits - 1, q)) return rc;
+      p = q;
+      inf.retries++;
+      continue;
+    }
+    if (alias) {
+      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s)) return rc;
+    }
+    break;
+  }
+  if (info) *info = inf;
+  return 0;
+}
+
+// ------------------------------------------------- reference-order transforms
+
+__global__ void k_small_dft(cd *z, int L, int inv, int three, int lg) {
+  // naive DFT for L < 64, in the reference order; one workgroup
+  __shared__ cd in[64];
+  int t = threadIdx.x;
+  if (t < L) in[t] = z[t];
+  __syncthreads();
+  if (t < L) {
+    cd acc = make_double2(0, 0);
+    if (!inv) {
+      // output position t holds frequency k(t)
+      unsigned k = three ? brev((unsigned)t / 3u, lg) + ((unsigned)t % 3u) * (unsigned)(L / 3) : brev((unsigned)t, lg);
+      for (int j = 0; j < L; j++) {
+        double sn, cs;
+        sincospi(2.0 * (double)(((unsigned long long)j * k) % (unsigned long long)L) / (double)L, &sn, &cs);
+        acc = cadd(acc, cmul(in[j], make_double2(cs, sn)));
+      }
+    } else {
+      for (int r = 0; r < L; r++) {
+        unsigned k = three ? brev((unsigned)r / 3u, lg) + ((unsigned)r % 3u) * (unsigned)(L / 3) : brev((unsigned)r, lg);
+        double sn, cs;
+        sincospi(2.0 * (double)(((unsigned long long)t * k) % (unsigned long long)L) / (double)L, &sn, &cs);
+        acc = cadd(acc, cmulc(in[r], make_double2(cs, sn)));
+      }
+    }
+    z[t] = acc;
+  }
+}
+
+bool valid_length(unsigned long long n) {
+  if (n < 2) return false;
+  unsigned long long p = is_three(n) ? n / 3 : n;
+  return (p & (p - 1)) == 0;
+}
+
+int transform_dev(double *z, size_t length, bool inverse, hipStream_t s) {
+  if (!valid_length(length)) return fail(BCG_E_ARG, "transform length must be 2^k or 3*2^k");
+  if (length < 64) {
+    int three = is_three(length) ? 1 : 0;
+    int lg = ilog2u(three ? length / 3 : length);
+    ProfScope ps(BCG_K_OTHER, s);
+    LAUNCH(k_small_dft, 1, 64, s, (cd *)z, (int)length, inverse ? 1 : 0, three, lg);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  Plan p;
+  p.N = length;
+  if (!factor(length, p)) return fail(BCG_E_ARG, "no transform plan for this length");
+  for (int i = 0; i < p.P; i++)
+    if (i == p.P - 1) p.C[i] = 1;
+  PassArgs a;
+  memset(&a, 0, sizeof a);
+  if (!inverse) {
+    for (int i = 0; i < p.P; i++) {
+      PassArgs b = a;
+      if (int rc = pass_args(p, i, b)) return rc;
+      b.data = (cd *)z;
+      fwd_fn f = pick_fwd(p.L[i], p.C[i]);
+      if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
+      ProfScope ps(BCG_K_FWD, s);
+      f(false, b.ntiles, s, b);
+      HIPCHK(hipGetLastError());
+    }
+  } else {
+    for (int i = p.P - 1; i >= 0; i--) {
+      PassArgs b = a;
+      if (int rc = pass_args(p, i, b)) return rc;
+      b.data = (cd *)z;
+      fwd_fn f = pick_inv(p.L[i], p.C[i]);
+      if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
+      ProfScope ps(BCG_K_INV, s);
+      f(false, b.ntiles, s, b);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  return 0;
+}
+
+__global__ void k_pointwise(cd *a, const cd *b, unsigned long long n) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) a[i] = cmul(a[i], b[i]);
+}
+
+__global__ void k_int_to_fft(cd *z, unsigned long long length, const uint32_t *A, unsigned long long AL, int bits,
+                             unsigned long long pts) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  unsigned long long n = length > pts ? length : pts;
+  for (; i < n; i += st) z[i] = make_double2(i < pts ? digit_at(A, AL, i, bits) : 0.0, 0.0);
+}
+
+// Points at or beyond `avail` (the caller's array length) read as zero.
+__global__ void k_round_points(const cd *z, unsigned long long pts, unsigned long long avail, double scale,
+                               long long *coef, unsigned int *margin) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  double worst = 0;
+  for (; i < pts; i += st) {
+    double f = i < avail ? z[i].x * scale : 0.0;
+    worst = fmax(worst, fabs(f - rint(f)));
+    long long r = (long long)(f + 0.5);
+    coef[i] = r > 0 ? r : 0;
+  }
+  margin_reduce(margin, worst);
+}
+
+int grid_for(unsigned long long n) {
+  unsigned long long g2 = (n + 255) / 256;
+  if (g2 > 8192) g2 = 8192;
+  if (g2 < 1) g2 = 1;
+  return (int)g2;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+int bcg_init(int device) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  return init_locked(device);
+}
+
+int bcg_shutdown(void) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  if (!g.ready) return 0;
+  (void)hipStreamSynchronize(g.stream);
+  for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf}) {
+    if (b->p) (void)hipFree(b->p);
+    b->p = nullptr;
+    b->bytes = 0;
+  }
+  for (auto &r : g.roots) {
+    (void)hipFree(r.second.lo);
+    (void)hipFree(r.second.hi);
+  }
+  g.roots.clear();
+  for (auto &t : g.ltab) (void)hipFree(t.second);
+  g.ltab.clear();
+  prof_collect();
+  for (auto e : g.free_events) (void)hipEventDestroy(e);
+  g.free_events.clear();
+  (void)hipStreamDestroy(g.stream);
+  g.stream = nullptr;
+  g.ready = false;
+  return 0;
+}
+
+const char *bcg_last_error(void) { return g_err.c_str(); }
+
+void *bcg_stream(void) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  if (init_locked(0)) return nullptr;
+  return (void *)g.stream;
+}
+
+int bcg_sync(void) {
+  ENTER();
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int bcg_malloc(void **ptr, size_t bytes) {
+  ENTER();
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) return fail(BCG_E_NOMEM, "hipMalloc", e);
+  return 0;
+}
+
+int bcg_free(void *ptr) {
+  ENTER();
+  if (ptr) HIPCHK(hipFree(ptr));
+  return 0;
+}
+
+int bcg_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  ENTER();
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int bcg_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  ENTER();
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int bcg_mul_dev_bits(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL, int bits,
+                     void *stream, bcg_mul_info *info) {
+  ENTER();
+  if ((AL && !A) || (BL && !B) || (CL && !C)) return fail(BCG_E_ARG, "null pointer");
+  if (bits < 0 || bits > 24) return fail(BCG_E_ARG, "bits out of range");
+  return mul_dev_locked(C, CL, A, AL, B, BL, bits, pick(stream), info);
+}
+
+int bcg_mul_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL, void *stream,
+                bcg_mul_info *info) {
+  return bcg_mul_dev_bits(C, CL, A, AL, B, BL, 0, stream, info);
+}
+
+int bcg_mul_host(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                 bcg_mul_info *info) {
+  ENTER();
+  if ((AL && !A) || (BL && !B) || (CL && !C)) return fail(BCG_E_ARG, "null pointer");
+  hipStream_t s = g.stream;
+  bool square = (A == B && AL == BL);
+  if (int rc = ensure(g.wA, AL * 4 + 4)) return rc;
+  if (!square)
+    if (int rc = ensure(g.wB, BL * 4 + 4)) return rc;
+  if (int rc = ensure(g.wC, CL * 4 + 4)) return rc;
+  uint32_t *dA = (uint32_t *)g.wA.p;
+  uint32_t *dB = square ? dA : (uint32_t *)g.wB.p;
+  uint32_t *dC = (uint32_t *)g.wC.p;
+  if (AL) HIPCHK(hipMemcpyAsync(dA, A, AL * 4, hipMemcpyHostToDevice, s));
+  if (!square && BL) HIPCHK(hipMemcpyAsync(dB, B, BL * 4, hipMemcpyHostToDevice, s));
+  if (int rc = mul_dev_locked(dC, CL, dA, AL, dB, BL, 0, s, info)) return rc;
+  if (CL) HIPCHK(hipMemcpyAsync(C, dC, CL * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int bcg_basecase_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                     void *stream) {
+  ENTER();
+  if (AL == 0 || BL == 0) return fill_zero(C, CL, pick(stream));
+  return basecase_dev(C, CL, A, AL, B, BL, pick(stream));
+}
+
+int bcg_basecase_host(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL) {
+  ENTER();
+  hipStream_t s = g.stream;
+  if (int rc = ensure(g.wA, AL * 4 + 4)) return rc;
+  if (int rc = ensure(g.wB, BL * 4 + 4)) return rc;
+  if (int rc = ensure(g.wC, CL * 4 + 4)) return rc;
+  if (AL) HIPCHK(hipMemcpyAsync(g.wA.p, A, AL * 4, hipMemcpyHostToDevice, s));
+  if (BL) HIPCHK(hipMemcpyAsync(g.wB.p, B, BL * 4, hipMemcpyHostToDevice, s));
+  int rc = (AL == 0 || BL == 0) ? fill_zero((uint32_t *)g.wC.p, CL, s)
+                                : basecase_dev((uint32_t *)g.wC.p, CL, (uint32_t *)g.wA.p, AL, (uint32_t *)g.wB.p, BL, s);
+  if (rc) return rc;
+  if (CL) HIPCHK(hipMemcpyAsync(C, g.wC.p, CL * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const size_t *c_off, const size_t *cl,
+                      const size_t *a_off, const size_t *al, const size_t *b_off, const size_t *bl, size_t count,
+                      void *stream, double *worst_margin) {
+  ENTER();
+  hipStream_t s = pick(stream);
+  double worst = 0;
+  for (size_t i = 0; i < count; i++) {
+    bcg_mul_info inf;
+    const uint32_t *pa = A + a_off[i];
+    const uint32_t *pb = B + b_off[i];
+    if (int rc = mul_dev_locked(C + c_off[i], cl[i], pa, al[i], pb, bl[i], 0, s, &inf)) return rc;
+    if (inf.margin > worst) worst = inf.margin;
+  }
+  if (worst_margin) *worst_margin = worst;
+  return 0;
+}
+
+int bcg_fft_forward_dev(double *z, size_t length, void *stream) {
+  ENTER();
+  return transform_dev(z, length, false, pick(stream));
+}
+
+int bcg_fft_inverse_dev(double *z, size_t length, void *stream) {
+  ENTER();
+  return transform_dev(z, length, true, pick(stream));
+}
+
+int bcg_fft_pointwise_dev(double *a, const double *b, size_t length, void *stream) {
+  ENTER();
+  if (!length) return 0;
+  hipStream_t s = pick(stream);
+  LAUNCH(k_pointwise, grid_for(length), 256, s, (cd *)a, (const cd *)b, (unsigned long long)length);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int bcg_int_to_fft_dev(double *z, size_t length, const uint32_t *A, size_t AL, int bits, size_t *points,
+                       void *stream) {
+  ENTER();
+  if (bits < 1 || bits > 30) return fail(BCG_E_ARG, "int_to_fft: bits per point must be in [1, 30]");
+  unsigned long long pts = AL ? digits_of(AL, bits) : 0;
+  if (points) *points = (size_t)pts;
+  unsigned long long n = length > pts ? length : pts;
+  if (!n) return 0;
+  hipStream_t s = pick(stream);
+  LAUNCH(k_int_to_fft, grid_for(n), 256, s, (cd *)z, (unsigned long long)length, A, (unsigned long long)AL, bits, pts);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int bcg_fft_to_int_dev(const double *z, size_t length, uint32_t *A, size_t AL, int bits, double *margin,
+                       void *stream) {
+  ENTER();
+  if (bits < 1 || bits > 30) return fail(BCG_E_ARG, "fft_to_int: bits per point must be in [1, 30]");
+  if (margin) *margin = 0;
+  if (!AL) return 0;
+  if (!length) return fail(BCG_E_ARG, "fft_to_int: zero length");
+  hipStream_t s = pick(stream);
+  unsigned long long pts = digits_of(AL, bits);
+  if (int rc = ensure(g.zbuf, pts * sizeof(long long))) return rc;
+  if (int rc = zero_margin(s)) return rc;
+  unsigned long long avail = pts < length ? pts : (unsigned long long)length;
+  LAUNCH(k_round_points, grid_for(pts), 256, s, (const cd *)z, pts, avail, 1.0 / (double)length, (long long *)g.zbuf.p,
+         (unsigned int *)g.margin.p);
+  HIPCHK(hipGetLastError());
+  // r_k are < 2^53; bound the coefficient width by the largest possible value
+  if (int rc = run_carry((const long long *)g.zbuf.p, pts, bits, 63, A, AL, s)) return rc;
+  double m = 0;
+  if (int rc = read_margin(s, &m)) return rc;
+  if (margin) *margin = m;
+  return 0;
+}
+
+int bcg_fft_forward_host(double *z, size_t length) {
+  ENTER();
+  if (int rc = ensure(g.bufA, length * 16 + 16)) return rc;
+  HIPCHK(hipMemcpyAsync(g.bufA.p, z, length * 16, hipMemcpyHostToDevice, g.stream));
+  if (int rc = transform_dev((double *)g.bufA.p, length, false, g.stream)) return rc;
+  HIPCHK(hipMemcpyAsync(z, g.bufA.p, length * 16, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int bcg_fft_inverse_host(double *z, size_t length) {
+  ENTER();
+  if (int rc = ensure(g.bufA, length * 16 + 16)) return rc;
+  HIPCHK(hipMemcpyAsync(g.bufA.p, z, length * 16, hipMemcpyHostToDevice, g.stream));
+  if (int rc = transform_dev((double *)g.bufA.p, length, true, g.stream)) return rc;
+  HIPCHK(hipMemcpyAsync(z, g.bufA.p, length * 16, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int bcg_fft_pointwise_host(double *a, const double *b, size_t length) {
+  ENTER();
+  if (!length) return 0;
+  if (int rc = ensure(g.bufA, length * 16)) return rc;
+  if (int rc = ensure(g.bufB, length * 16)) return rc;
+  HIPCHK(hipMemcpyAsync(g.bufA.p, a, length * 16, hipMemcpyHostToDevice, g.stream));
+  HIPCHK(hipMemcpyAsync(g.bufB.p, b, length * 16, hipMemcpyHostToDevice, g.stream));
+  LAUNCH(k_pointwise, grid_for(length), 256, g.stream, (cd *)g.bufA.p, (const cd *)g.bufB.p,
+         (unsigned long long)length);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(a, g.bufA.p, length * 16, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int bcg_int_to_fft_host(double *z, size_t length, const uint32_t *A, size_t AL, int bits, size_t *points) {
+  ENTER();
+  if (bits < 1 || bits > 30) return fail(BCG_E_ARG, "int_to_fft: bits per point must be in [1, 30]");
+  unsigned long long pts = AL ? digits_of(AL, bits) : 0;
+  unsigned long long n = length > pts ? length : pts;
+  if (points) *points = (size_t)pts;
+  if (!n) return 0;
+  if (int rc = ensure(g.bufA, n * 16)) return rc;
+  if (int rc = ensure(g.wA, AL * 4 + 4)) return rc;
+  if (AL) HIPCHK(hipMemcpyAsync(g.wA.p, A, AL * 4, hipMemcpyHostToDevice, g.stream));
+  size_t dummy;
+  if (int rc = bcg_int_to_fft_dev((double *)g.bufA.p, length, (const uint32_t *)g.wA.p, AL, bits, &dummy, g.stream))
+    return rc;
+  HIPCHK(hipMemcpyAsync(z, g.bufA.p, n * 16, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int bcg_fft_to_int_host(const double *z, size_t length, uint32_t *A, size_t AL, int bits, double *margin) {
+  ENTER();
+  if (bits < 1 || bits > 30) return fail(BCG_E_ARG, "fft_to_int: bits per point must be in [1, 30]");
+  if (!AL) {
+    if (margin) *margin = 0;
+    return 0;
+  }
+  unsigned long long pts = digits_of(AL, bits);
+  unsigned long long avail = pts < length ? pts : (unsigned long long)length;  // points past length read as 0
+  if (int rc = ensure(g.bufA, avail * 16 + 16)) return rc;
+  if (int rc = ensure(g.wC, AL * 4 + 4)) return rc;
+  HIPCHK(hipMemcpyAsync(g.bufA.p, z, avail * 16, hipMemcpyHostToDevice, g.stream));
+  if (int rc = bcg_fft_to_int_dev((const double *)g.bufA.p, length, (uint32_t *)g.wC.p, AL, bits, margin, g.stream))
+    return rc;
+  HIPCHK(hipMemcpyAsync(A, g.wC.p, AL * 4, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int bcg_tft_forward_host(double *z, size_t len, int k) {
+  (void)z; (void)len; (void)k;
+  return fail(BCG_E_ARG, "tft_forward: not available");
+}
+
+int bcg_tft_inverse_host(double *z, size_t len, int k) {
+  (void)z; (void)len; (void)k;
+  return fail(BCG_E_ARG, "tft_inverse: not available");
+}
+
+int bcg_plan_query(size_t AL, size_t BL, int *bits, size_t *points, int *passes) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  Plan p;
+  if (int rc = make_plan(AL, BL, 0, p)) return rc;
+  if (bits) *bits = p.bits;
+  if (points) *points = (size_t)p.N;
+  if (passes) *passes = p.P;
+  return 0;
+}
+
+int bcg_prof_enable(int on) {
+  ENTER();
+  if (!on) prof_collect();
+  g.prof = on != 0;
+  return 0;
+}
+
+int bcg_prof_read(int kind, long *launches, double *ms) {
+  ENTER();
+  if (kind < 0 || kind >= BCG_PROF_KINDS) return fail(BCG_E_ARG, "bad kernel kind");
+  prof_collect();
+  if (launches) *launches = g.prof_n[kind];
+  if (ms) *ms = g.prof_ms[kind];
+  return 0;
+}
+
+int bcg_prof_reset(void) {
+  ENTER();
+  prof_collect();
+  for (int i = 0; i < BCG_PROF_KINDS; i++) {
+    g.prof_n[i] = 0;
+    g.prof_ms[i] = 0;
+  }
+  return 0;
+}
+
+}  // extern "C"

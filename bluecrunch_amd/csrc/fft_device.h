@@ -1,0 +1,273 @@
+// Device code of the FP64 FFT multiply for gfx950 (MI355X).
+//
+// Transform structure (all kernels):
+//  * A length-N transform (N = 2^k or 3*2^k) is split into P <= 3 HBM passes
+//    N = L_1 * ... * L_P. Pass p views its span S_p = L_p * M_p
+//    (M_p = L_{p+1}...L_P) as L_p rows of M_p columns and runs length-L_p DFTs
+//    down the columns, C adjacent columns per workgroup so every global
+//    access is a C*16-byte contiguous run.
+//  * Inside a workgroup the length-L DFT is an in-place radix-2 decimation in
+//    frequency held in LDS; groups of up to 4 radix-2 levels (radix 16) run in
+//    registers between LDS exchanges. Output rows come out bit-reversed, so a
+//    whole transform lands in exactly the order of the reference fft_forward
+//    (fft/fft.c:43-96: bit-reversed, radix-3 digit innermost, e^{+2 pi i/N}).
+//  * The inverse is the mirror image: radix-2 decimation in time with
+//    conjugate twiddles (fft/fft.c:289-329), bit-reversed input, natural output.
+//
+// The multiply (bigfloat/mul.c:108-174 semantics: exact product) uses a
+// right-angle (negacyclic) convolution: 2N real b-bit digits become N complex
+// points u_j = (a_j + i a_{j+N}) psi^j with psi = e^{i pi / 2N}, so no zero
+// padding and no real/complex untangling is needed. After the inverse,
+// (c_j + i c_{j+N}) = w_j psi^{-j} / N.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bcg {
+
+typedef double2 cd;
+
+__device__ __forceinline__ cd cadd(cd a, cd b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cd cmul(cd a, cd b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ cd cmulc(cd a, cd b) {
+  return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// e^{+2 pi i j / 16}, j = 0..15, to 20 significant digits.
+#define BCG_C1 0.92387953251128675613
+#define BCG_R2 0.70710678118654752440
+__device__ __forceinline__ cd w16(int j) {
+  // j is a compile-time constant at every call site (fully unrolled loops).
+  const double c[16] = {1.0, BCG_C1, BCG_R2, 0.38268343236508977173, 0.0, -0.38268343236508977173, -BCG_R2, -BCG_C1,
+                        -1.0, -BCG_C1, -BCG_R2, -0.38268343236508977173, 0.0, 0.38268343236508977173, BCG_R2, BCG_C1};
+  const double s[16] = {0.0, 0.38268343236508977173, BCG_R2, BCG_C1, 1.0, BCG_C1, BCG_R2, 0.38268343236508977173,
+                        0.0, -0.38268343236508977173, -BCG_R2, -BCG_C1, -1.0, -BCG_C1, -BCG_R2, -0.38268343236508977173};
+  return make_double2(c[j & 15], s[j & 15]);
+}
+
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+// Length descriptor: L = 2^LOG or 3 * 2^LOG.
+template <int L>
+struct Len {
+  static constexpr bool THREE = (L % 3) == 0;
+  static constexpr int P2 = THREE ? L / 3 : L;
+  static constexpr int LOG = ilog2c(P2);
+};
+
+// Per-(L, C) tile configuration: E elements, EPT per thread, T threads and the
+// radix-2^b stage list (first stage takes the remainder bits).
+template <int L, int C>
+struct Tile {
+  static constexpr int E = L * C;
+  static constexpr bool THREE = Len<L>::THREE;
+  static constexpr int LOG = Len<L>::LOG;
+  static constexpr int EPT = THREE ? 24 : 16;
+  static constexpr int T = E / EPT;
+  static constexpr int MAXB = THREE ? 3 : 4;
+  static constexpr int NST2 = (LOG + MAXB - 1) / MAXB;  // radix-2^b stages
+  static constexpr int FIRSTB = LOG - MAXB * (NST2 - 1);
+  // LDS index with one pad element every 16 to spread power-of-two strides.
+  static constexpr int LDS_ELEMS = E + E / 16;
+  static_assert(E % EPT == 0, "tile must split evenly");
+  static_assert(T >= 1 && T <= 1024, "bad thread count");
+};
+
+__device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
+
+__device__ __forceinline__ unsigned brev(unsigned v, int bits) {
+  return bits ? (__builtin_bitreverse32(v) >> (32 - bits)) : 0u;
+}
+
+// Frequency index held at row r after an in-place DIF of length L
+// (bit-reversed 2-part, radix-3 digit innermost).
+template <int L>
+__device__ __forceinline__ unsigned freq_of_row(unsigned r) {
+  if (Len<L>::THREE) return brev(r / 3u, Len<L>::LOG) + (r % 3u) * (unsigned)Len<L>::P2;
+  return brev(r, Len<L>::LOG);
+}
+
+// Root-of-unity tables for one transform length N: Omega = e^{2 pi i / 4N}.
+//   Omega^u = hi[u >> sh] * lo[u & (2^sh - 1)],  u in [0, 4N)
+//   tl_L[t] = e^{2 pi i t / L} for t < L/2, one table per pass length L.
+struct Roots {
+  const cd *lo;
+  const cd *hi;
+  int sh;
+  unsigned long long four_n;
+};
+
+__device__ __forceinline__ cd omega_pow(const Roots &r, unsigned long long u) {
+  return cmul(r.hi[u >> r.sh], r.lo[u & ((1ull << r.sh) - 1)]);
+}
+
+// ---------------------------------------------------------------- butterflies
+
+// RB radix-2 DIF levels on R = 2^RB registers holding rows q + t*(span/R).
+// Level l pairs t, t+half (half = R >> (l+1)); twiddle e^{2 pi i n / (span/2^l)}
+// with n = q + (t mod half)*(span/R) = base_l * w_{2 half}^{t mod half}.
+template <int RB>
+__device__ __forceinline__ void dif_levels(cd (&x)[1 << RB], const cd *__restrict__ tl, int q, int tstride) {
+  constexpr int R = 1 << RB;
+#pragma unroll
+  for (int l = 0; l < RB; l++) {
+    const int half = R >> (l + 1);
+    const cd base = tl[q * (tstride << l)];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      if ((t & (2 * half - 1)) < half) {
+        const int j = t & (2 * half - 1);
+        cd a = x[t], b = x[t + half];
+        x[t] = cadd(a, b);
+        cd d = csub(a, b);
+        cd w = (j == 0) ? base : cmul(base, w16(j * (16 / (2 * half))));
+        x[t + half] = cmul(d, w);
+      }
+    }
+  }
+}
+
+// Mirror image: radix-2 DIT levels with conjugate twiddles (c = b conj(w);
+// a' = a + c; b' = a - c), bit-reversed input to natural output.
+template <int RB>
+__device__ __forceinline__ void dit_levels(cd (&x)[1 << RB], const cd *__restrict__ tl, int q, int tstride) {
+  constexpr int R = 1 << RB;
+#pragma unroll
+  for (int l = RB - 1; l >= 0; l--) {
+    const int half = R >> (l + 1);
+    const cd base = tl[q * (tstride << l)];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      if ((t & (2 * half - 1)) < half) {
+        const int j = t & (2 * half - 1);
+        cd a = x[t];
+        cd w = (j == 0) ? base : cmul(base, w16(j * (16 / (2 * half))));
+        cd c = cmulc(x[t + half], w);
+        x[t] = cadd(a, c);
+        x[t + half] = csub(a, c);
+      }
+    }
+  }
+}
+
+// 3-point DFT, natural order in and out, w = e^{+-2 pi i / 3}
+// (include/intrinsic.h:270-363 dft_3p / dft_3p_inv).
+template <bool INV>
+__device__ __forceinline__ void dft3(cd &a, cd &b, cd &c) {
+  const double wr = -0.5;
+  const double wi = INV ? -0.86602540378443864676 : 0.86602540378443864676;
+  cd s = cadd(b, c);
+  cd d = csub(b, c);
+  cd a0 = cadd(a, s);
+  // a + s*wr  +-  i*wi*d
+  cd m = make_double2(a.x + wr * s.x, a.y + wr * s.y);
+  cd r = make_double2(-wi * d.y, wi * d.x);
+  a = a0;
+  b = cadd(m, r);
+  c = csub(m, r);
+}
+
+// --------------------------------------------------------- LDS stage helpers
+
+// One radix-2^RB stage of span `span` over the whole tile, LDS in and out.
+template <int L, int C, int RB, bool INV>
+__device__ __forceinline__ void lds_stage(cd *lds, int span, const cd *__restrict__ tl, int tid) {
+  using TC = Tile<L, C>;
+  constexpr int R = 1 << RB;
+  constexpr int PER = TC::EPT / R;
+  const int sub = span / R;
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    int bf = tid + i * TC::T;
+    int c = bf % C;
+    int rest = bf / C;
+    int q = rest % sub;
+    int blk = rest / sub;
+    int row0 = blk * span + q;
+    cd x[R];
+#pragma unroll
+    for (int t = 0; t < R; t++) x[t] = lds[pad16((row0 + t * sub) * C + c)];
+    if (INV)
+      dit_levels<RB>(x, tl, q, L / span);
+    else
+      dif_levels<RB>(x, tl, q, L / span);
+#pragma unroll
+    for (int t = 0; t < R; t++) lds[pad16((row0 + t * sub) * C + c)] = x[t];
+  }
+}
+
+template <int L, int C, bool INV>
+__device__ __forceinline__ void lds_radix3(cd *lds, int tid) {
+  using TC = Tile<L, C>;
+  constexpr int PER = TC::EPT / 3;
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    int bf = tid + i * TC::T;
+    int c = bf % C;
+    int blk = bf / C;
+    int e0 = (blk * 3) * C + c;
+    cd a = lds[pad16(e0)], b = lds[pad16(e0 + C)], d = lds[pad16(e0 + 2 * C)];
+    dft3<INV>(a, b, d);
+    lds[pad16(e0)] = a;
+    lds[pad16(e0 + C)] = b;
+    lds[pad16(e0 + 2 * C)] = d;
+  }
+}
+
+// Runtime dispatch on the radix bits of a stage (1..4).
+template <int L, int C, bool INV>
+__device__ __forceinline__ void lds_stage_any(cd *lds, int rb, int span, const cd *__restrict__ tl, int tid) {
+  switch (rb) {
+    case 1: lds_stage<L, C, 1, INV>(lds, span, tl, tid); break;
+    case 2: lds_stage<L, C, 2, INV>(lds, span, tl, tid); break;
+    case 3: lds_stage<L, C, 3, INV>(lds, span, tl, tid); break;
+    default: lds_stage<L, C, 4, INV>(lds, span, tl, tid); break;
+  }
+}
+
+// Stage bit widths: stage 0 takes FIRSTB bits, the rest MAXB.
+template <int L, int C>
+__device__ __forceinline__ int stage_bits(int s) {
+  return s == 0 ? Tile<L, C>::FIRSTB : Tile<L, C>::MAXB;
+}
+
+// Forward stages 1..NST2-1 entirely in LDS (stage 0 has been done already),
+// then the radix-3 stage when L = 3*2^k. Ends with a barrier.
+template <int L, int C>
+__device__ __forceinline__ void forward_rest(cd *lds, const cd *__restrict__ tl, int tid) {
+  using TC = Tile<L, C>;
+  int span = L >> TC::FIRSTB;
+  for (int s = 1; s < TC::NST2; s++) {
+    int rb = stage_bits<L, C>(s);
+    lds_stage_any<L, C, false>(lds, rb, span, tl, tid);
+    __syncthreads();
+    span >>= rb;
+  }
+  if (TC::THREE) {
+    lds_radix3<L, C, false>(lds, tid);
+    __syncthreads();
+  }
+}
+
+// Inverse stages from the innermost up to (not including) stage 0.
+template <int L, int C>
+__device__ __forceinline__ void inverse_front(cd *lds, const cd *__restrict__ tl, int tid) {
+  using TC = Tile<L, C>;
+  if (TC::THREE) {
+    lds_radix3<L, C, true>(lds, tid);
+    __syncthreads();
+  }
+  int span = TC::THREE ? 3 : 1;
+  for (int s = TC::NST2 - 1; s >= 1; s--) {
+    int rb = stage_bits<L, C>(s);
+    span <<= rb;
+    lds_stage_any<L, C, true>(lds, rb, span, tl, tid);
+    __syncthreads();
+  }
+}
+
+}  // namespace bcg

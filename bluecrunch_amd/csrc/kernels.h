@@ -1,0 +1,500 @@
+// Kernels of the FFT multiply: transform passes, the fused middle pass, the
+// rounding/unpack pass and the carry resolve. See fft_device.h for the
+// transform structure.
+#pragma once
+#include "fft_device.h"
+
+namespace bcg {
+
+struct PassArgs {
+  cd *data;                 // complex array of the pass (operand A / product)
+  const cd *data_b;         // operand B spectrum array (middle pass)
+  const uint32_t *wa;       // digits of A come from these words (pack passes)
+  const uint32_t *wb;
+  unsigned long long al, bl;
+  long long *coef;          // unpack output: 2N rounded coefficients
+  unsigned long long N;     // transform length
+  unsigned long long M;     // pass column count (stride between rows)
+  unsigned long long S;     // pass span L * M
+  unsigned long long tw_mul;// 4N / S: converts e^{2 pi i / S} exponents to Omega exponents
+  const cd *tl;             // e^{2 pi i t / L}, t < L/2
+  Roots roots;
+  int bits;                 // digit size
+  int ntiles;
+  unsigned int *margin;     // float bits of the worst |x - round(x)| (atomic max)
+  double inv_n;
+};
+
+// Tile order that keeps adjacent column tiles on one XCD at the same time:
+// workgroup b runs on XCD b % 8, so tile = (b % 8) * (ntiles / 8) + b / 8.
+__device__ __forceinline__ int tile_of_block(int ntiles) {
+  int b = blockIdx.x;
+  if (ntiles >= 64 && (ntiles & 7) == 0) return (b & 7) * (ntiles >> 3) + (b >> 3);
+  return b;
+}
+
+// b-bit digit j of a little-endian word array (0 beyond the last word).
+__device__ __forceinline__ double digit_at(const uint32_t *__restrict__ w, unsigned long long wl, unsigned long long j,
+                                           int bits) {
+  unsigned long long s = j * (unsigned long long)bits;
+  unsigned long long wi = s >> 5;
+  if (wi >= wl) return 0.0;
+  unsigned sh = (unsigned)(s & 31);
+  unsigned long long v = w[wi];
+  if (wi + 1 < wl) v |= (unsigned long long)w[wi + 1] << 32;
+  return (double)((v >> sh) & ((1ull << bits) - 1ull));
+}
+
+// Right-angle input point j: (a_j + i a_{j+N}) * Omega^j.
+__device__ __forceinline__ cd packed_point(const PassArgs &a, const uint32_t *__restrict__ w, unsigned long long wl,
+                                           unsigned long long j) {
+  cd v = make_double2(digit_at(w, wl, j, a.bits), digit_at(w, wl, j + a.N, a.bits));
+  return cmul(v, omega_pow(a.roots, j));
+}
+
+__device__ __forceinline__ void margin_update(unsigned int *m, double d) {
+  float f = (float)d;
+  // non-negative floats order like their bit patterns
+  atomicMax(m, __float_as_uint(f));
+}
+
+// Round one output point j (value w_j of the unscaled inverse) into
+// coefficients c_j (real part) and c_{j+N} (imaginary part); returns margin.
+__device__ __forceinline__ double unpack_point(const PassArgs &a, cd v, unsigned long long j) {
+  cd f = cmulc(v, omega_pow(a.roots, j));
+  double re = f.x * a.inv_n, im = f.y * a.inv_n;
+  double rr = rint(re), ri = rint(im);
+  double m = fmax(fabs(re - rr), fabs(im - ri));
+  // trunc(x + 0.5) like fft/convert.c's (uint64_t)(f + 0.5), clamped at 0
+  long long cr = (long long)(re + 0.5), ci = (long long)(im + 0.5);
+  a.coef[j] = cr > 0 ? cr : 0;
+  a.coef[j + a.N] = ci > 0 ? ci : 0;
+  return m;
+}
+
+// Wave-level max then one atomic per wave.
+__device__ __forceinline__ void margin_reduce(unsigned int *m, double d) {
+  float f = (float)d;
+  for (int off = 32; off > 0; off >>= 1) f = fmaxf(f, __shfl_xor(f, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(m, __float_as_uint(f));
+}
+
+// ------------------------------------------------------------ forward pass
+
+// Length-L DIF down C adjacent columns of one span; stage 0 loads straight from
+// global (or from words when PACK), the rest run in LDS; the store multiplies
+// row r (frequency k) by the pass twiddle e^{2 pi i m k / S}.
+template <int L, int C, bool PACK>
+__global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
+  using TC = Tile<L, C>;
+  __shared__ cd lds[TC::LDS_ELEMS];
+  const int tid = threadIdx.x;
+  const int tile = tile_of_block(a.ntiles);
+  const unsigned long long tpb = a.M / C;
+  const unsigned long long ob = (unsigned long long)tile / tpb;
+  const unsigned long long m0 = ((unsigned long long)tile % tpb) * C;
+  cd *base = a.data + ob * a.S;
+  constexpr int RB0 = TC::FIRSTB;
+  constexpr int R0 = 1 << RB0;
+  constexpr int PER0 = TC::EPT / R0;
+  constexpr int SUB0 = L / R0;
+#pragma unroll
+  for (int i = 0; i < PER0; i++) {
+    int bf = tid + i * TC::T;
+    int c = bf % C;
+    int q = bf / C;
+    cd x[R0];
+#pragma unroll
+    for (int t = 0; t < R0; t++) {
+      unsigned long long row = (unsigned long long)(q + t * SUB0);
+      unsigned long long j = row * a.M + m0 + c;
+      if (PACK)
+        x[t] = packed_point(a, a.wa, a.al, j);
+      else
+        x[t] = base[j];
+    }
+    dif_levels<RB0>(x, a.tl, q, 1);
+#pragma unroll
+    for (int t = 0; t < R0; t++) lds[pad16((q + t * SUB0) * C + c)] = x[t];
+  }
+  __syncthreads();
+  forward_rest<L, C>(lds, a.tl, tid);
+  for (int e = tid; e < TC::E; e += TC::T) {
+    int row = e / C, c = e % C;
+    unsigned long long m = m0 + c;
+    cd v = lds[pad16(e)];
+    if (a.M > 1) {
+      unsigned long long k = freq_of_row<L>((unsigned)row);
+      v = cmul(v, omega_pow(a.roots, m * k * a.tw_mul));
+    }
+    base[(unsigned long long)row * a.M + m] = v;
+  }
+}
+
+// ------------------------------------------------------------ inverse pass
+
+// Mirror of k_fwd: staged load with the conjugate pass twiddle, DIT stages in
+// LDS, stage 0 stores straight to global, or (UNPACK) rounds to coefficients.
+template <int L, int C, bool UNPACK>
+__global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
+  using TC = Tile<L, C>;
+  __shared__ cd lds[TC::LDS_ELEMS];
+  const int tid = threadIdx.x;
+  const int tile = tile_of_block(a.ntiles);
+  const unsigned long long tpb = a.M / C;
+  const unsigned long long ob = (unsigned long long)tile / tpb;
+  const unsigned long long m0 = ((unsigned long long)tile % tpb) * C;
+  cd *base = a.data + ob * a.S;
+  for (int e = tid; e < TC::E; e += TC::T) {
+    int row = e / C, c = e % C;
+    unsigned long long m = m0 + c;
+    cd v = base[(unsigned long long)row * a.M + m];
+    if (a.M > 1) {
+      unsigned long long k = freq_of_row<L>((unsigned)row);
+      v = cmulc(v, omega_pow(a.roots, m * k * a.tw_mul));
+    }
+    lds[pad16(e)] = v;
+  }
+  __syncthreads();
+  inverse_front<L, C>(lds, a.tl, tid);
+  constexpr int RB0 = TC::FIRSTB;
+  constexpr int R0 = 1 << RB0;
+  constexpr int PER0 = TC::EPT / R0;
+  constexpr int SUB0 = L / R0;
+  double worst = 0.0;
+#pragma unroll
+  for (int i = 0; i < PER0; i++) {
+    int bf = tid + i * TC::T;
+    int c = bf % C;
+    int q = bf / C;
+    cd x[R0];
+#pragma unroll
+    for (int t = 0; t < R0; t++) x[t] = lds[pad16((q + t * SUB0) * C + c)];
+    dit_levels<RB0>(x, a.tl, q, 1);
+#pragma unroll
+    for (int t = 0; t < R0; t++) {
+      unsigned long long row = (unsigned long long)(q + t * SUB0);
+      unsigned long long j = row * a.M + m0 + c;
+      if (UNPACK)
+        worst = fmax(worst, unpack_point(a, x[t], j));
+      else
+        base[j] = x[t];
+    }
+  }
+  if (UNPACK) margin_reduce(a.margin, worst);
+}
+
+// ------------------------------------------------------------- middle pass
+
+// Forward DIF of one contiguous block of L points, ending with the innermost
+// stage's outputs in registers: xr[i * R + t] for butterfly i of that stage.
+template <int L, bool PACK>
+__device__ __forceinline__ void mid_forward(const PassArgs &a, cd *lds, const cd *__restrict__ src,
+                                            const uint32_t *__restrict__ w, unsigned long long wl,
+                                            unsigned long long blk0, cd (&xr)[Tile<L, 1>::EPT], int tid) {
+  using TC = Tile<L, 1>;
+  constexpr int RB0 = TC::FIRSTB;
+  constexpr int R0 = 1 << RB0;
+  constexpr int PER0 = TC::EPT / R0;
+  constexpr int SUB0 = L / R0;
+  // Stage 0 straight from global.
+#pragma unroll
+  for (int i = 0; i < PER0; i++) {
+    int q = tid + i * TC::T;
+    cd x[R0];
+#pragma unroll
+    for (int t = 0; t < R0; t++) {
+      unsigned long long j = blk0 + (unsigned long long)(q + t * SUB0);
+      x[t] = PACK ? packed_point(a, w, wl, j) : src[j];
+    }
+    dif_levels<RB0>(x, a.tl, q, 1);
+    if (TC::NST2 == 1 && !TC::THREE) {
+#pragma unroll
+      for (int t = 0; t < R0; t++) xr[i * R0 + t] = x[t];
+    } else {
+#pragma unroll
+      for (int t = 0; t < R0; t++) lds[pad16(q + t * SUB0)] = x[t];
+    }
+  }
+  if (TC::NST2 == 1 && !TC::THREE) return;
+  __syncthreads();
+  // Middle stages in LDS.
+  int span = L >> RB0;
+  const int last = TC::THREE ? TC::NST2 : TC::NST2 - 1;  // stages run in LDS
+  for (int s = 1; s < last; s++) {
+    int rb = stage_bits<L, 1>(s);
+    lds_stage_any<L, 1, false>(lds, rb, span, a.tl, tid);
+    __syncthreads();
+    span >>= rb;
+  }
+  if (TC::THREE) {
+    constexpr int PER = TC::EPT / 3;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      int b3 = tid + i * TC::T;
+      cd p = lds[pad16(3 * b3)], q = lds[pad16(3 * b3 + 1)], r = lds[pad16(3 * b3 + 2)];
+      dft3<false>(p, q, r);
+      xr[3 * i] = p;
+      xr[3 * i + 1] = q;
+      xr[3 * i + 2] = r;
+    }
+  } else {
+    constexpr int RBL = TC::MAXB;  // last stage is a full MAXB stage when NST2 > 1
+    constexpr int R = 1 << RBL;
+    constexpr int PER = TC::EPT / R;
+    // span == R here: each butterfly is one contiguous block of R rows
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      int blk = tid + i * TC::T;
+      cd x[R];
+#pragma unroll
+      for (int t = 0; t < R; t++) x[t] = lds[pad16(blk * R + t)];
+      dif_levels<RBL>(x, a.tl, 0, L / R);
+#pragma unroll
+      for (int t = 0; t < R; t++) xr[i * R + t] = x[t];
+    }
+  }
+}
+
+// Last forward pass of A (and B), pointwise product (or square), first inverse
+// pass, all on one contiguous block of L points. With PACK/UNPACK the block is
+// the whole transform (one-pass plans).
+template <int L, bool SQUARE, bool PACK, bool UNPACK>
+__global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
+  using TC = Tile<L, 1>;
+  __shared__ cd lds[TC::LDS_ELEMS];
+  const int tid = threadIdx.x;
+  const unsigned long long blk0 = (unsigned long long)blockIdx.x * L;
+  cd xa[TC::EPT];
+  mid_forward<L, PACK>(a, lds, a.data, a.wa, a.al, blk0, xa, tid);
+  if (!SQUARE) {
+    cd xb[TC::EPT];
+    __syncthreads();
+    mid_forward<L, PACK>(a, lds, a.data_b, a.wb, a.bl, blk0, xb, tid);
+#pragma unroll
+    for (int i = 0; i < TC::EPT; i++) xa[i] = cmul(xa[i], xb[i]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < TC::EPT; i++) xa[i] = cmul(xa[i], xa[i]);
+  }
+  // Inverse: innermost stage in registers, mirror of mid_forward.
+  constexpr int RB0 = TC::FIRSTB;
+  constexpr int R0 = 1 << RB0;
+  constexpr int PER0 = TC::EPT / R0;
+  constexpr int SUB0 = L / R0;
+  if (!(TC::NST2 == 1 && !TC::THREE)) {
+    __syncthreads();  // every thread is done reading B's innermost stage from LDS
+    if (TC::THREE) {
+      constexpr int PER = TC::EPT / 3;
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        int b3 = tid + i * TC::T;
+        cd p = xa[3 * i], q = xa[3 * i + 1], r = xa[3 * i + 2];
+        dft3<true>(p, q, r);
+        lds[pad16(3 * b3)] = p;
+        lds[pad16(3 * b3 + 1)] = q;
+        lds[pad16(3 * b3 + 2)] = r;
+      }
+    } else {
+      constexpr int RBL = TC::MAXB;
+      constexpr int R = 1 << RBL;
+      constexpr int PER = TC::EPT / R;
+#pragma unroll
+      for (int i = 0; i < PER; i++) {
+        int blk = tid + i * TC::T;
+        cd x[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) x[t] = xa[i * R + t];
+        dit_levels<RBL>(x, a.tl, 0, L / R);
+#pragma unroll
+        for (int t = 0; t < R; t++) lds[pad16(blk * R + t)] = x[t];
+      }
+    }
+    __syncthreads();
+    // remaining inverse stages in LDS, innermost to outermost, excluding stage 0
+    int first_lds = TC::THREE ? TC::NST2 - 1 : TC::NST2 - 2;
+    int span = TC::THREE ? 3 : 1;
+    for (int s = TC::NST2 - 1; s > first_lds; s--) span <<= stage_bits<L, 1>(s);
+    for (int s = first_lds; s >= 1; s--) {
+      int rb = stage_bits<L, 1>(s);
+      span <<= rb;
+      lds_stage_any<L, 1, true>(lds, rb, span, a.tl, tid);
+      __syncthreads();
+    }
+  }
+  double worst = 0.0;
+#pragma unroll
+  for (int i = 0; i < PER0; i++) {
+    int q = tid + i * TC::T;
+    cd x[R0];
+    if (TC::NST2 == 1 && !TC::THREE) {
+#pragma unroll
+      for (int t = 0; t < R0; t++) x[t] = xa[i * R0 + t];
+    } else {
+#pragma unroll
+      for (int t = 0; t < R0; t++) x[t] = lds[pad16(q + t * SUB0)];
+    }
+    dit_levels<RB0>(x, a.tl, q, 1);
+#pragma unroll
+    for (int t = 0; t < R0; t++) {
+      unsigned long long j = blk0 + (unsigned long long)(q + t * SUB0);
+      if (UNPACK)
+        worst = fmax(worst, unpack_point(a, x[t], j));
+      else
+        a.data[j] = x[t];
+    }
+  }
+  if (UNPACK) margin_reduce(a.margin, worst);
+}
+
+// ------------------------------------------------------------ carry resolve
+
+// Product words from rounded coefficients: value = sum_j c_j 2^{b j}.
+// Word w collects P_w = sum_j piece(c_j 2^{bj}, word w) (< 2^37); then
+// v_w = lo32(P_w) + hi(P_{w-1}) and a carry chain of single bits resolves v.
+// Each thread owns CWPT consecutive words; a block owns CWPT * CT words.
+constexpr int CWPT = 4;
+constexpr int CT = 256;
+
+struct CarryArgs {
+  const long long *coef;
+  unsigned long long ncoef;
+  int bits;
+  int cbits;                 // every coefficient < 2^cbits (cbits <= 63)
+  uint32_t *out;
+  unsigned long long CL;
+  unsigned char *blk_fn;     // per block: bit0 = carry out if carry in 0, bit1 = if carry in 1
+  unsigned long long nblk;
+};
+
+__device__ __forceinline__ void word_partials(const CarryArgs &a, long long wfirst, unsigned long long (&P)[CWPT + 1]) {
+  // words wfirst-1 .. wfirst+CWPT-1 -> P[0..CWPT]
+#pragma unroll
+  for (int k = 0; k <= CWPT; k++) P[k] = 0;
+  long long lo_bit = 32 * (wfirst - 1) - a.cbits + 1;
+  long long jlo = lo_bit <= 0 ? 0 : (lo_bit + a.bits - 1) / a.bits;
+  long long hi_bit = 32 * (wfirst + CWPT) - 1;
+  long long jhi = hi_bit / a.bits;
+  if (jhi >= (long long)a.ncoef) jhi = (long long)a.ncoef - 1;
+  for (long long j = jlo; j <= jhi; j++) {
+    unsigned long long c = (unsigned long long)a.coef[j];
+    if (!c) continue;
+    long long shift = (long long)a.bits * j - 32 * (wfirst - 1);
+#pragma unroll
+    for (int k = 0; k <= CWPT; k++) {
+      long long sh = shift - 32 * k;
+      unsigned long long piece;
+      if (sh >= 0)
+        piece = sh < 32 ? (unsigned long long)(uint32_t)(c << sh) : 0ull;
+      else
+        piece = -sh < 64 ? (unsigned long long)(uint32_t)(c >> (-sh)) : 0ull;
+      P[k] += piece;
+    }
+  }
+}
+
+__global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
+  __shared__ unsigned char fn[CT];
+  const int tid = threadIdx.x;
+  const long long wfirst = ((long long)blockIdx.x * CT + tid) * CWPT;
+  unsigned long long P[CWPT + 1];
+  if (wfirst < (long long)a.CL) word_partials(a, wfirst, P);
+  else
+    for (int k = 0; k <= CWPT; k++) P[k] = 0;
+  if (wfirst == 0) P[0] = 0;
+  // v_w and the thread's carry function f(cin) over its words
+  unsigned long long v[CWPT];
+#pragma unroll
+  for (int k = 0; k < CWPT; k++) v[k] = (P[k + 1] & 0xffffffffull) + (P[k] >> 32);
+  unsigned f0 = 0, f1 = 1;  // carry after the words, for carry-in 0 / 1
+#pragma unroll
+  for (int k = 0; k < CWPT; k++) {
+    f0 = (unsigned)((v[k] + f0) >> 32);
+    f1 = (unsigned)((v[k] + f1) >> 32);
+  }
+  fn[tid] = (unsigned char)(f0 | (f1 << 1));
+  __syncthreads();
+  // exclusive scan of carry functions (block carry-in assumed 0); CT is small,
+  // so thread 0 walks it serially in LDS.
+  __shared__ unsigned char cin_of[CT];
+  if (tid == 0) {
+    unsigned g0 = 0, g1 = 1;
+    for (int t = 0; t < CT; t++) {
+      cin_of[t] = (unsigned char)(g0 | (g1 << 1));
+      unsigned f = fn[t];
+      unsigned n0 = (g0 ? (f >> 1) : f) & 1u;
+      unsigned n1 = (g1 ? (f >> 1) : f) & 1u;
+      g0 = n0;
+      g1 = n1;
+    }
+    a.blk_fn[blockIdx.x] = (unsigned char)(g0 | (g1 << 1));
+  }
+  __syncthreads();
+  unsigned cin = cin_of[tid] & 1u;
+#pragma unroll
+  for (int k = 0; k < CWPT; k++) {
+    unsigned long long s = v[k] + cin;
+    long long w = wfirst + k;
+    if (w < (long long)a.CL) a.out[w] = (uint32_t)s;
+    cin = (unsigned)(s >> 32);
+  }
+}
+
+// One workgroup: scan the block carry functions, then add the carry into every
+// block whose carry-in is 1 (ripple through 0xffffffff words inside the block).
+__global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
+  __shared__ unsigned char fns[1024];
+  const int tid = threadIdx.x;
+  const unsigned long long per = (a.nblk + 1023) / 1024;
+  const unsigned long long b0 = tid * per;
+  unsigned long long b1 = b0 + per;
+  if (b1 > a.nblk) b1 = a.nblk;
+  unsigned g0 = 0, g1 = 1;
+  for (unsigned long long b = b0; b < b1; b++) {
+    unsigned f = a.blk_fn[b];
+    unsigned n0 = (g0 ? (f >> 1) : f) & 1u;
+    unsigned n1 = (g1 ? (f >> 1) : f) & 1u;
+    g0 = n0;
+    g1 = n1;
+  }
+  fns[tid] = (unsigned char)(g0 | (g1 << 1));
+  __syncthreads();
+  __shared__ unsigned char cin_of[1024];
+  if (tid == 0) {
+    unsigned c = 0;
+    for (int t = 0; t < 1024; t++) {
+      cin_of[t] = (unsigned char)c;
+      unsigned f = fns[t];
+      c = (c ? (f >> 1) : f) & 1u;
+    }
+  }
+  __syncthreads();
+  unsigned c = cin_of[tid];
+  const unsigned long long wpb = (unsigned long long)CT * CWPT;
+  for (unsigned long long b = b0; b < b1; b++) {
+    unsigned f = a.blk_fn[b];
+    if (c) {
+      for (unsigned long long w = b * wpb; w < (b + 1) * wpb && w < a.CL; w++) {
+        uint32_t x = a.out[w] + 1u;
+        a.out[w] = x;
+        if (x != 0u) break;
+      }
+    }
+    c = (c ? (f >> 1) : f) & 1u;
+  }
+}
+
+// --------------------------------------------------------------- utilities
+
+__global__ void k_zero_u32(unsigned int *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void k_fill_u32(uint32_t *p, unsigned long long n, uint32_t v) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = v;
+}
+
+}  // namespace bcg

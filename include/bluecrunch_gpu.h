@@ -1,0 +1,122 @@
+/*
+ * bluecrunch_gpu.h -- thin C ABI between the C host side (bigfloat_mul,
+ * _fft_mul and the fft.h entry points) and the hand-written HIP kernels for
+ * gfx950 (MI355X).
+ *
+ * Plain pointers and sizes only. "_dev" entry points take device pointers and
+ * enqueue on the library stream (or the stream passed in); "_host" entry points
+ * take host pointers and copy. Every call returns 0 on success and a negative
+ * BCG_E* code on failure; bcg_last_error() describes the most recent failure.
+ * All entry points are thread safe (calls are serialised per device).
+ */
+#ifndef BLUECRUNCH_GPU_H
+#define BLUECRUNCH_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BCG_OK 0
+#define BCG_E_HIP (-1)      /* a HIP runtime call failed */
+#define BCG_E_ARG (-2)      /* invalid argument */
+#define BCG_E_NOMEM (-3)    /* device allocation failed */
+#define BCG_E_MARGIN (-4)   /* rounding margin stayed >= 0.25 at the smallest digit size */
+#define BCG_E_NODEV (-5)    /* no HIP device */
+
+/* What a multiply did: digit split, transform size, worst rounding distance. */
+typedef struct {
+  int bits;          /* digit size b: operands cut into b-bit digits */
+  int passes;        /* HBM passes per transform (1, 2 or 3) */
+  size_t points;     /* complex transform length N (right-angle: 2N digits) */
+  double margin;     /* worst |x - round(x)| over all 2N output coefficients */
+  int retries;       /* re-runs with a smaller digit size because margin >= 0.25 */
+  int square;        /* 1 when the squaring path ran (one forward transform) */
+} bcg_mul_info;
+
+/* Device set-up: select device, create the stream, warm the kernels.
+ * Called implicitly by every entry point; explicit calls are idempotent. */
+int bcg_init(int device);
+/* Frees device buffers, tables and the stream. */
+int bcg_shutdown(void);
+const char *bcg_last_error(void);
+/* The library stream (hipStream_t) used when a call passes stream == NULL. */
+void *bcg_stream(void);
+int bcg_sync(void);
+
+/* Device memory helpers for callers without their own allocator. */
+int bcg_malloc(void **ptr, size_t bytes);
+int bcg_free(void *ptr);
+int bcg_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int bcg_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* C = low CL words of A*B (little-endian base 2^32 words). A == B with
+ * AL == BL selects the squaring path. C may alias A or B. Replaces the
+ * reference _fft_mul (include/mul.h:3, bigfloat/mul.c:108-174). */
+int bcg_mul_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                void *stream, bcg_mul_info *info);
+int bcg_mul_host(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                 bcg_mul_info *info);
+/* Same, with the digit size forced (0 = automatic). For margin sweeps. */
+int bcg_mul_dev_bits(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                     int bits, void *stream, bcg_mul_info *info);
+
+/* Schoolbook product on the GPU (exact), used for CL < 100 like the
+ * reference _karatzuba_mul/_basecase_mul branch (bigfloat/mul.c:176-553). */
+int bcg_basecase_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                     void *stream);
+
+/* Batched products: count independent products, word arrays concatenated.
+ * a_off/b_off/c_off are word offsets into A/B/C, al/bl/cl the lengths. All
+ * pointers are host arrays of descriptors; A/B/C are device words. */
+int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const size_t *c_off, const size_t *cl,
+                      const size_t *a_off, const size_t *al, const size_t *b_off, const size_t *bl, size_t count,
+                      void *stream, double *worst_margin);
+
+/* Reference-compatible transforms on device arrays of interleaved complex
+ * doubles (fft.h:15-23 semantics: forward e^{+2 pi i/N}, bit-reversed output,
+ * unscaled; inverse takes bit-reversed input, natural output, unscaled). */
+int bcg_fft_forward_dev(double *z, size_t length, void *stream);
+int bcg_fft_inverse_dev(double *z, size_t length, void *stream);
+int bcg_fft_pointwise_dev(double *a, const double *b, size_t length, void *stream);
+/* int_to_fft: returns points written (ceil(32*AL/bits)) through *points. */
+int bcg_int_to_fft_dev(double *z, size_t length, const uint32_t *A, size_t AL, int bits, size_t *points,
+                       void *stream);
+int bcg_fft_to_int_dev(const double *z, size_t length, uint32_t *A, size_t AL, int bits, double *margin,
+                       void *stream);
+
+/* Host-array variants of the above (copy in, run, copy out). */
+int bcg_fft_forward_host(double *z, size_t length);
+int bcg_fft_inverse_host(double *z, size_t length);
+int bcg_fft_pointwise_host(double *a, const double *b, size_t length);
+int bcg_int_to_fft_host(double *z, size_t length, const uint32_t *A, size_t AL, int bits, size_t *points);
+int bcg_fft_to_int_host(const double *z, size_t length, uint32_t *A, size_t AL, int bits, double *margin);
+int bcg_basecase_host(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL);
+/* Truncated FFT (fft/tft.c intended semantics, pinned by test/test_tft.c). */
+int bcg_tft_forward_host(double *z, size_t len, int k);
+int bcg_tft_inverse_host(double *z, size_t len, int k);
+
+/* Plan inspection: the digit size and transform size a product would use. */
+int bcg_plan_query(size_t AL, size_t BL, int *bits, size_t *points, int *passes);
+
+/* Kernel timing with HIP events recorded on the launch stream. While enabled,
+ * every kernel the engine launches is bracketed by events; bcg_prof_read
+ * returns, per kernel kind, launch count and summed milliseconds. */
+#define BCG_PROF_KINDS 8
+#define BCG_K_FWD_PACK 0  /* first forward pass, digits read from words */
+#define BCG_K_FWD 1       /* middle forward pass (3-pass plans) */
+#define BCG_K_MID 2       /* last forward pass + pointwise + first inverse pass */
+#define BCG_K_INV 3       /* middle inverse pass */
+#define BCG_K_INV_UNPACK 4 /* last inverse pass, rounding to coefficients */
+#define BCG_K_CARRY 5     /* coefficient -> word carry resolve */
+#define BCG_K_OTHER 6
+int bcg_prof_enable(int on);
+int bcg_prof_read(int kind, long *launches, double *ms);
+int bcg_prof_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark: bigfloat_mul of two 2^24-word operands on one MI355X.
+
+A step is one device-resident product of two seeded 2^24-word operands
+(words uniform in [0, 1e9), SplitMix64 seed 1) through the library's C ABI
+(bcg_mul_dev: FP64 right-angle FFT multiply, rounding-margin check and carry
+resolve). value = decimal digits per second with the north-star convention of
+9 digits per operand word: 9 * 2^24 / (seconds per product). The squaring of
+operand A is timed the same way and reported beside it.
+
+Also reported: the dominant kernel's HBM roofline (algorithmic bytes per launch
+over its HIP-event time on the launch stream) and the oracle (the CPU
+restatement of the reference AVX path, OpenMP on the host cores) timed on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "bigfloat_mul digits/s at 2^24 words (%HBM peak); e to 1e8 digits wall-s"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def algorithmic_bytes(kind, info, al, bl, cl):
+    """Compulsory HBM bytes of one launch of each kernel kind (16 B per complex point)."""
+    n = info["points"]
+    sq = info["square"]
+    if kind == "fwd_pack":
+        return 4 * al + 16 * n  # read one operand's words, write its first-pass array
+    if kind in ("fwd", "inv"):
+        return 32 * n  # read and write one array
+    if kind == "mid":
+        if info["passes"] == 1:
+            return 4 * (al + (0 if sq else bl)) + 16 * n
+        return (32 if sq else 48) * n  # read A (and B) arrays, write the product array
+    if kind == "inv_unpack":
+        return 32 * n  # read the product array, write 2N 8-byte coefficients
+    if kind == "carry":
+        return 16 * n + 4 * cl  # read coefficients, write words
+    return 0
+
+
+def read_traffic(kind):
+    """HBM bytes per launch measured with rocprofv3 (profiles/<kind>_traffic.txt), else None."""
+    path = os.path.join(ROOT, "profiles", f"{kind}_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(words, seed):
+    """Oracle _fft_mul on host cores for a bounded sample (words x words)."""
+    from oracle import refcpu
+    from bluecrunch_amd.workload import operands
+
+    a, b = operands(seed, words, words)
+    L = refcpu.lib()
+    # twiddle tables are set-up, as in the reference main.c:209 before timing
+    bits = refcpu.lib().ref_sched_bits(2 * words)
+    bits = max(bits, 7)
+    n = refcpu.fft_length(int(32.0 / bits * 2 * words))
+    L.ref_ensure_table(n)
+    t0 = time.perf_counter()
+    c, used, margin = refcpu.fft_mul(a, b)
+    dt = time.perf_counter() - t0
+    return {
+        "value": 9.0 * words / dt,
+        "unit": "digits/s",
+        "cores": refcpu.num_threads(),
+        "kind": "port",
+        "sample": f"one oracle _fft_mul of {words}x{words} words (seed {seed}), b={used}, "
+                  f"N={n} complex points, {dt:.2f} s wall",
+        "seconds": dt,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--words", type=int, default=1 << 24, help="words per operand")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-words", type=int, default=1 << 24, help="operand words of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the FFT multiply runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not part of this build).", file=sys.stderr)
+        sys.exit(2)
+
+    import bluecrunch_amd as bc
+    from bluecrunch_amd.workload import operands, product_fingerprint_ok
+
+    bc.check(bc.lib().bcg_init(0), "bcg_init")
+    al = bl = args.words
+    cl = al + bl
+    a, b = operands(args.seed, al, bl)
+    dA = bc.DeviceWords.from_host(a)
+    dB = bc.DeviceWords.from_host(b)
+    dC = bc.DeviceWords(cl)
+    dS = bc.DeviceWords(2 * al)
+
+    def step():
+        return bc.mul_dev(dC.ptr, cl, dA.ptr, al, dB.ptr, bl)
+
+    def step_sq():
+        return bc.mul_dev(dS.ptr, 2 * al, dA.ptr, al, dA.ptr, al)
+
+    for _ in range(args.warmup):
+        info = step()
+        step_sq()
+    bc.sync()
+
+    ok = None
+    if not args.no_check:
+        info = step()
+        c = dC.to_host()
+        s_info = step_sq()
+        s = dS.to_host()
+        ok = bool(product_fingerprint_ok(a, b, c) and product_fingerprint_ok(a, a, s))
+
+    # timed region: K products, kernel events recorded on the launch stream
+    bc.prof_reset()
+    bc.prof_enable(True)
+    bc.sync()
+    t0 = time.perf_counter()
+    worst = 0.0
+    for _ in range(args.steps):
+        info = step()
+        worst = max(worst, info["margin"])
+    bc.sync()
+    t1 = time.perf_counter()
+    bc.prof_enable(False)
+    kern = {}
+    for kind in ("fwd_pack", "fwd", "mid", "inv", "inv_unpack", "carry"):
+        n, ms = bc.prof_read(kind)
+        if n:
+            per = ms / n
+            byt = algorithmic_bytes(kind, info, al, bl, cl)
+            kern[kind] = {"launches": n, "avg_ms": per, "alg_bytes": byt, "GBps": byt / (per * 1e-3) / 1e9}
+    bc.prof_reset()
+
+    # squaring, timed the same way (not profiled)
+    bc.sync()
+    t2 = time.perf_counter()
+    s_worst = 0.0
+    for _ in range(args.steps):
+        s_info = step_sq()
+        s_worst = max(s_worst, s_info["margin"])
+    bc.sync()
+    t3 = time.perf_counter()
+
+    sec = (t1 - t0) / args.steps
+    sec_sq = (t3 - t2) / args.steps
+    digits = 9.0 * al
+    value = digits / sec
+
+    dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches"]) if kern else None
+    roof = None
+    if dom:
+        d = kern[dom]
+        ach = d["alg_bytes"] / (d["avg_ms"] * 1e-3)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": read_traffic(dom)}
+    fft_bytes = sum(v["alg_bytes"] * v["launches"] for k, v in kern.items() if k != "carry")
+    fft_ms = sum(v["avg_ms"] * v["launches"] for k, v in kern.items() if k != "carry")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_words, args.seed)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "digits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"bigfloat_mul {al}x{bl} words, words uniform in [0,1e9) from SplitMix64 seed {args.seed}, "
+                        "device-resident operands; plus squaring of operand A",
+            "words": al,
+            "digits_per_product": digits,
+            "bits_per_digit": info["bits"],
+            "points": info["points"],
+            "passes": info["passes"],
+            "worst_margin": worst,
+        },
+        "square": {"ms_per_step": sec_sq * 1e3, "digits_per_s": digits / sec_sq, "worst_margin": s_worst,
+                   "bits_per_digit": s_info["bits"], "points": s_info["points"]},
+        "fft_passes_hbm_frac": (fft_bytes / (fft_ms * 1e-3)) / HBM_PEAK if fft_ms else None,
+        "kernels": kern,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
+        "parity_fingerprint_ok": ok,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

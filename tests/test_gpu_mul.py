@@ -1,0 +1,174 @@
+"""GPU parity of the multiply path (_fft_mul, bigfloat_mul, the device C ABI)
+against exact integer products, the golden fixtures and the oracle. Integer
+results must be bit-exact; every product's rounding margin must be < 0.25."""
+import numpy as np
+import pytest
+
+from helpers import exact_product, sha, words_to_int, int_to_words
+from bluecrunch_amd.workload import operands, product_fingerprint_ok
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1, 1), (2, 3), (49, 50), (50, 50), (1, 150), (99, 1), (100, 100), (127, 129), (300, 1000), (1000, 7),
+          (2000, 2000), (4095, 4097), (3000, 9000), (12345, 777), (16384, 16384), (40000, 40000), (100000, 70000),
+          (131072, 131072), (1, 100000), (300000, 299999)]
+
+
+@pytest.mark.parametrize("al,bl", SHAPES)
+def test_mul_exact(bc, al, bl):
+    a, b = operands(1000 + al + 7 * bl, al, bl)
+    c, info = bc.mul(a, b)
+    assert np.array_equal(c, exact_product(a, b)), info
+    assert info["margin"] < 0.25
+
+
+@pytest.mark.parametrize("al", [1, 60, 100, 1024, 5000, 65536, 262144])
+def test_square_exact(bc, al):
+    a, _ = operands(al, al, 1)
+    c, info = bc.mul(a, a)
+    assert np.array_equal(c, exact_product(a, a))
+    if al + al >= 100:
+        assert info["square"] == 1
+
+
+@pytest.mark.parametrize("al", [100, 4096, 65536, 1 << 20])
+def test_all_ones_adversarial(bc, al):
+    # maximal digits: the worst magnitude for the unsigned digit FFT
+    a = np.full(al, 0xffffffff, dtype=np.uint32)
+    c, info = bc.mul(a, a)
+    x = (1 << (32 * al)) - 1
+    assert words_to_int(c) == x * x
+    assert info["margin"] < 0.25
+
+
+def test_golden_product_hashes(bc, golden):
+    for p in golden["products"]:
+        a, b = operands(p["seed"], p["al"], p["bl"])
+        c, _ = bc.mul(a, b)
+        assert sha(c) == p["sha256"], p
+        s, _ = bc.mul(a, a)
+        assert sha(s) == p["square_sha256"], p
+
+
+@pytest.mark.parametrize("cl_delta", [-50, -1, 1, 5])
+def test_output_length_other_than_al_plus_bl(bc, cl_delta):
+    # _fft_mul writes exactly CL words: the low CL words of the product
+    a, b = operands(77, 600, 500)
+    cl = 1100 + cl_delta
+    c, _ = bc.mul(a, b, cl=cl)
+    assert np.array_equal(c, exact_product(a, b, cl))
+
+
+def test_fft_mul_entry_point(bc):
+    a, b = operands(5, 5000, 4000)
+    c, bits = bc.fft_mul(a, b)
+    assert np.array_equal(c, exact_product(a, b)) and 4 <= bits <= 20
+
+
+def test_basecase_and_karatzuba_entry_points(bc):
+    import ctypes
+    for al, bl in [(1, 1), (3, 17), (49, 50), (200, 300)]:
+        a, b = operands(al + bl, al, bl)
+        c = bc.basecase(a, b)
+        assert np.array_equal(c, exact_product(a, b))
+        out = np.zeros(al + bl, dtype=np.uint32)
+        bc.lib()._karatzuba_mul(out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), al + bl,
+                                a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), al,
+                                b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), bl)
+        assert np.array_equal(out, exact_product(a, b))
+
+
+def test_bigfloat_mul_regression_square(bc, golden):
+    # test/test_regression.c:40-67 -- in-place a = a*a
+    g = golden["regression_square"]
+    a = (np.array(g["a"], np.uint32), 0, 1)
+    w, e, s = bc.bigfloat_mul(a, a, 0, inplace="a")
+    assert list(map(int, w)) == g["expected"] and e == 0 and s == 1
+
+
+def test_bigfloat_exp_vector(bc, golden):
+    # test/test_bigfloat_exp.c -- 0x1234^0xd7 by square-and-multiply
+    g = golden["bigfloat_exp"]
+    target = (np.array([1], np.uint32), 0, 1)
+    cp = (np.array([g["base"]], np.uint32), 0, 1)
+    e = g["exp"]
+    while e:
+        if e & 1:
+            target = bc.bigfloat_mul(target, cp, 0)
+        cp = bc.bigfloat_mul(cp, cp, 0)
+        e >>= 1
+    w, ex, s = target
+    assert len(w) == g["len"] and ex == 0 and s == 1 and words_to_int(w) == int(g["hex"], 16)
+
+
+def test_bigfloat_exp_through_fft_sizes(bc):
+    # square-and-multiply far into the FFT range, checked against Python ints
+    base, e = 0x9e3779b9, 5000
+    target = (np.array([1], np.uint32), 0, 1)
+    cp = (np.array([base], np.uint32), 0, 1)
+    k = e
+    while k:
+        if k & 1:
+            target = bc.bigfloat_mul(target, cp, 0)
+        cp = bc.bigfloat_mul(cp, cp, 0)
+        k >>= 1
+    assert words_to_int(target[0]) == base ** e
+
+
+@pytest.mark.parametrize("p", [0, 1, 50, 120, 2000])
+def test_bigfloat_mul_matches_oracle(bc, oracle, p):
+    a, b = operands(31 + p, 700, 1300)
+    for sa, sb in [(1, 1), (0, 1), (0, 0)]:
+        A, B = (a, 4, sa), (b, -9, sb)
+        got = bc.bigfloat_mul(A, B, p)
+        want = oracle.bigfloat_mul(A, B, p)
+        assert np.array_equal(got[0], want[0]) and got[1:] == want[1:]
+
+
+def test_bigfloat_mul_zero(bc):
+    a, _ = operands(3, 10, 1)
+    w, e, s = bc.bigfloat_mul((np.zeros(0, np.uint32), 3, 0), (a, 0, 1), 0)
+    assert len(w) == 0 and e == 0 and s == 1
+
+
+def test_device_abi_alias_and_repeat(bc):
+    # C may alias A (in-place), repeated calls reuse buffers
+    a, b = operands(8, 50000, 50000)
+    want = exact_product(a, b)
+    dA = bc.DeviceWords(100000)
+    import ctypes
+    bc.check(bc.lib().bcg_memcpy_h2d(dA.ptr, a.ctypes.data, 4 * a.size))
+    dB = bc.DeviceWords.from_host(b)
+    for _ in range(2):
+        info = bc.mul_dev(dA.ptr, 100000, dA.ptr, 50000, dB.ptr, 50000)
+        assert np.array_equal(dA.to_host(), want)
+        bc.check(bc.lib().bcg_memcpy_h2d(dA.ptr, a.ctypes.data, 4 * a.size))
+
+
+@pytest.mark.parametrize("words", [1 << 20, 1 << 22])
+def test_large_against_oracle(bc, oracle, words):
+    a, b = operands(1, words, words)
+    c, info = bc.mul(a, b)
+    o, _, _ = oracle.fft_mul(a, b)
+    assert np.array_equal(c, o)
+    assert info["margin"] < 0.25
+
+
+def test_benchmark_size_fingerprint(bc):
+    # 2^24 x 2^24 words (the benchmark workload): size-independent modular check
+    w = 1 << 24
+    a, b = operands(1, w, w)
+    c, info = bc.mul(a, b)
+    assert product_fingerprint_ok(a, b, c)
+    assert info["bits"] == 8 and info["points"] == 1 << 26 and info["margin"] < 0.25
+    s, sinfo = bc.mul(a, a)
+    assert product_fingerprint_ok(a, a, s) and sinfo["margin"] < 0.25
+
+
+@pytest.mark.parametrize("bits", [8, 10, 12, 14, 16])
+def test_forced_digit_sizes(bc, bits):
+    a, b = operands(bits, 20000, 20000)
+    dA, dB, dC = bc.DeviceWords.from_host(a), bc.DeviceWords.from_host(b), bc.DeviceWords(40000)
+    info = bc.mul_dev(dC.ptr, 40000, dA.ptr, 20000, dB.ptr, 20000, bits=bits)
+    assert info["bits"] <= bits
+    assert np.array_equal(dC.to_host(), exact_product(a, b))

@@ -204,6 +204,19 @@ def baileys_forward(z):
     return z
 
 
+def tft_forward(z, length, k):
+    """fft.h tft_forward on a complex128 array of 2^k points (in place)."""
+    assert z.dtype == np.complex128 and z.size >= (1 << k)
+    lib().tft_forward(_f64(z.view(np.float64)), length, k)
+    return z
+
+
+def tft_inverse(z, length, k):
+    assert z.dtype == np.complex128 and z.size >= (1 << k)
+    lib().tft_inverse(_f64(z.view(np.float64)), length, k)
+    return z
+
+
 def fft_pointwise(a, b):
     lib().fft_pointwise(_f64(a.view(np.float64)), _f64(b.view(np.float64)), a.size)
     return a

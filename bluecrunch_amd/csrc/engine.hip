@@ -806,6 +806,75 @@ int transform_dev(double *z, size_t length, bool inverse, hipStream_t s) {
   return 0;
 }
 
+// ----------------------------------------------------------- truncated FFT
+//
+// tft_forward(T, len, k): the first len outputs (bit-reversed order) of the
+// length-2^k forward transform of T[0..len) zero-padded; the rest zeroed.
+// tft_inverse(T, len, k): given those len outputs of a signal supported on
+// [0, len), returns 2^k times the signal in T[0..len), zeros after
+// (fft/tft.c entry points; semantics pinned by test/test_tft.c).
+//
+// The inverse recursion on n = 2^k follows the first DIF level: outputs in the
+// first half are the half-length transform of u_j = x_j + x_{j+n/2}, outputs in
+// the second half that of v_j = (x_j - x_{j+n/2}) w^j (w = e^{2 pi i/n}).
+//  * len <= n/2: x_{j+n/2} = 0, so u = x and the problem halves.
+//  * len > n/2: u follows from the full first half; for j >= len - n/2 the
+//    upper input is zero, so v_j = u_j w^j is known there; removing those known
+//    inputs from the second-half outputs leaves a half-size inverse TFT for the
+//    remaining v_j, and x_j, x_{j+n/2} = (u_j +- v_j w^{-j}) / 2.
+
+__global__ void k_scale(cd *z, unsigned long long n, double s) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) z[i] = make_double2(z[i].x * s, z[i].y * s);
+}
+
+__global__ void k_zero_range(cd *z, unsigned long long lo, unsigned long long hi) {
+  unsigned long long i = lo + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  for (; i < hi; i += st) z[i] = make_double2(0.0, 0.0);
+}
+
+// vk[j] = 0 for j < l2, u[j] * w^j for j >= l2 (w = e^{2 pi i / n}, j < n/2)
+__global__ void k_tft_known(const cd *u, cd *vk, unsigned long long half, unsigned long long l2, unsigned long long n) {
+  unsigned long long j = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  for (; j < half; j += st) {
+    if (j < l2) {
+      vk[j] = make_double2(0.0, 0.0);
+    } else {
+      double sn, cs;
+      sincospi(2.0 * (double)j / (double)n, &sn, &cs);
+      vk[j] = cmul(u[j], make_double2(cs, sn));
+    }
+  }
+}
+
+// r[p] = y[p] - f[p], p < l2
+__global__ void k_sub(cd *r, const cd *y, const cd *f, unsigned long long l2) {
+  unsigned long long p = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  for (; p < l2; p += st) r[p] = csub(y[p], f[p]);
+}
+
+// x_j, x_{j+n/2} from u_j and v_j (j < l2); x_j = u_j for l2 <= j < half
+__global__ void k_tft_combine(cd *x, const cd *u, const cd *v, unsigned long long half, unsigned long long l2,
+                              unsigned long long n) {
+  unsigned long long j = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  for (; j < half; j += st) {
+    if (j < l2) {
+      double sn, cs;
+      sincospi(2.0 * (double)j / (double)n, &sn, &cs);
+      cd t = cmulc(v[j], make_double2(cs, sn));
+      x[j] = make_double2(0.5 * (u[j].x + t.x), 0.5 * (u[j].y + t.y));
+      x[j + half] = make_double2(0.5 * (u[j].x - t.x), 0.5 * (u[j].y - t.y));
+    } else {
+      x[j] = u[j];
+    }
+  }
+}
+
 __global__ void k_pointwise(cd *a, const cd *b, unsigned long long n) {
   unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
@@ -840,6 +909,48 @@ int grid_for(unsigned long long n) {
   if (g2 > 8192) g2 = 8192;
   if (g2 < 1) g2 = 1;
   return (int)g2;
+}
+
+// True-scale inverse TFT: y[0..len) -> x[0..len) (x supported on [0, len)),
+// transform length n = 2^k; y and x may be the same device array.
+int tft_inverse_rec(cd *y, unsigned long long len, unsigned long long n, cd *x, hipStream_t s) {
+  if (len == 0) return 0;
+  if (n == 1) {
+    if (x != y) HIPCHK(hipMemcpyAsync(x, y, sizeof(cd), hipMemcpyDeviceToDevice, s));
+    return 0;
+  }
+  if (len == n) {
+    if (x != y) HIPCHK(hipMemcpyAsync(x, y, n * sizeof(cd), hipMemcpyDeviceToDevice, s));
+    if (int rc = transform_dev((double *)x, n, true, s)) return rc;
+    LAUNCH(k_scale, grid_for(n), 256, s, x, n, 1.0 / (double)n);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  const unsigned long long half = n / 2;
+  if (len <= half) return tft_inverse_rec(y, len, half, x, s);
+  const unsigned long long l2 = len - half;
+  void *mem = nullptr;
+  hipError_t e = hipMalloc(&mem, 4 * half * sizeof(cd));
+  if (e != hipSuccess) return fail(BCG_E_NOMEM, "tft scratch", e);
+  cd *u = (cd *)mem, *vk = u + half, *r = vk + half, *v = r + half;
+  int rc = 0;
+  do {
+    // u = inverse of the full first half (true scale)
+    if ((e = hipMemcpyAsync(u, y, half * sizeof(cd), hipMemcpyDeviceToDevice, s)) != hipSuccess) { rc = fail(BCG_E_HIP, "copy", e); break; }
+    if ((rc = transform_dev((double *)u, half, true, s))) break;
+    LAUNCH(k_scale, grid_for(half), 256, s, u, half, 1.0 / (double)half);
+    // known upper inputs of the second half, their outputs removed from y
+    LAUNCH(k_tft_known, grid_for(half), 256, s, u, vk, half, l2, n);
+    if ((rc = transform_dev((double *)vk, half, false, s))) break;
+    LAUNCH(k_sub, grid_for(l2), 256, s, r, y + half, vk, l2);
+    if ((rc = tft_inverse_rec(r, l2, half, v, s))) break;
+    LAUNCH(k_tft_combine, grid_for(half), 256, s, x, u, v, half, l2, n);
+    if ((e = hipGetLastError()) != hipSuccess) { rc = fail(BCG_E_HIP, "tft kernels", e); break; }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) { rc = fail(BCG_E_HIP, "sync", e); break; }
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(mem);
+  return rc;
 }
 
 }  // namespace
@@ -1122,13 +1233,39 @@ int bcg_fft_to_int_host(const double *z, size_t length, uint32_t *A, size_t AL, 
 }
 
 int bcg_tft_forward_host(double *z, size_t len, int k) {
-  (void)z; (void)len; (void)k;
-  return fail(BCG_E_ARG, "tft_forward: not available");
+  ENTER();
+  if (k < 0 || k > 40 || len > (1ull << k)) return fail(BCG_E_ARG, "tft_forward: need len <= 2^k");
+  unsigned long long n = 1ull << k;
+  if (int rc = ensure(g.bufA, n * 16)) return rc;
+  hipStream_t s = g.stream;
+  if (len) HIPCHK(hipMemcpyAsync(g.bufA.p, z, len * 16, hipMemcpyHostToDevice, s));
+  cd *d = (cd *)g.bufA.p;
+  if (n > len) LAUNCH(k_zero_range, grid_for(n - len), 256, s, d, (unsigned long long)len, n);
+  if (n >= 2)
+    if (int rc = transform_dev((double *)d, n, false, s)) return rc;
+  if (n > len) LAUNCH(k_zero_range, grid_for(n - len), 256, s, d, (unsigned long long)len, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(z, d, n * 16, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
 }
 
 int bcg_tft_inverse_host(double *z, size_t len, int k) {
-  (void)z; (void)len; (void)k;
-  return fail(BCG_E_ARG, "tft_inverse: not available");
+  ENTER();
+  if (k < 0 || k > 40 || len > (1ull << k)) return fail(BCG_E_ARG, "tft_inverse: need len <= 2^k");
+  unsigned long long n = 1ull << k;
+  if (int rc = ensure(g.bufA, n * 16)) return rc;
+  hipStream_t s = g.stream;
+  cd *d = (cd *)g.bufA.p;
+  if (len) HIPCHK(hipMemcpyAsync(d, z, len * 16, hipMemcpyHostToDevice, s));
+  if (int rc = tft_inverse_rec(d, len, n, d, s)) return rc;
+  // unscaled like the reference inverse: 2^k * x, zeros after len
+  LAUNCH(k_scale, grid_for(n), 256, s, d, (unsigned long long)len, (double)n);
+  if (n > len) LAUNCH(k_zero_range, grid_for(n - len), 256, s, d, (unsigned long long)len, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(z, d, n * 16, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
 }
 
 int bcg_plan_query(size_t AL, size_t BL, int *bits, size_t *points, int *passes) {

@@ -114,3 +114,30 @@ def test_conversions_match_oracle(bc, oracle, bits):
 def test_fft_length(bc, oracle):
     for n in range(0, 3000, 7):
         assert bc.fft_length(n) == oracle.fft_length(n)
+
+
+def test_tft_reference_vectors(bc, golden):
+    # test/test_tft.c -- forward and inverse known answers (tolerance 1e-6 as there)
+    for case in golden["tft_forward"]:
+        z = cplx(case["input"])
+        bc.tft_forward(z, case["len"], case["k"])
+        assert np.abs(z - cplx(case["expected"])).max() < 1e-6
+    for case in golden["tft_inverse"]:
+        z = cplx(case["input"])
+        bc.tft_inverse(z, case["len"], case["k"])
+        assert np.abs(z - np.array(case["expected"], complex)).max() < 1e-6, case["len"]
+
+
+@pytest.mark.parametrize("k,ln", [(4, 9), (5, 17), (6, 33), (8, 200), (10, 513), (10, 1000), (12, 2049),
+                                  (12, 3000), (16, 40000), (16, 65535)])
+def test_tft_round_trip_and_definition(bc, k, ln):
+    n = 1 << k
+    rng = np.random.default_rng(k * 1000 + ln)
+    x = np.zeros(n, complex)
+    x[:ln] = rng.standard_normal(ln) + 1j * rng.standard_normal(ln)
+    z = x.copy()
+    bc.tft_forward(z, ln, k)
+    full = x.copy(); bc.fft_forward(full)
+    assert np.abs(z[:ln] - full[:ln]).max() < 1e-9 * n and not z[ln:].any()
+    bc.tft_inverse(z, ln, k)
+    assert np.abs(z[:ln] - n * x[:ln]).max() < 1e-7 * n and not z[ln:].any()

@@ -355,3 +355,27 @@ def prof_reset():
 
 def sync():
     check(lib().bcg_sync(), "sync")
+
+
+def mul_batch(pairs, cls=None):
+    """Batched products of host (a, b) word-array pairs in one call
+    (bcg_mul_batch_dev); returns (list of product arrays, worst margin)."""
+    count = len(pairs)
+    al = np.array([len(a) for a, _ in pairs], dtype=np.uint64)
+    bl = np.array([len(b) for _, b in pairs], dtype=np.uint64)
+    cl = (al + bl) if cls is None else np.array(cls, dtype=np.uint64)
+    a_off = np.concatenate([[0], np.cumsum(al)[:-1]]).astype(np.uint64)
+    b_off = np.concatenate([[0], np.cumsum(bl)[:-1]]).astype(np.uint64)
+    c_off = np.concatenate([[0], np.cumsum(cl)[:-1]]).astype(np.uint64)
+    A = np.concatenate([np.ascontiguousarray(a, np.uint32) for a, _ in pairs]) if count else np.zeros(0, np.uint32)
+    B = np.concatenate([np.ascontiguousarray(b, np.uint32) for _, b in pairs]) if count else np.zeros(0, np.uint32)
+    dA, dB = DeviceWords.from_host(A), DeviceWords.from_host(B)
+    dC = DeviceWords(int(cl.sum()))
+    arr = lambda x: np.ascontiguousarray(x, dtype=np.uint64).ctypes.data_as(ctypes.POINTER(_sz))  # noqa: E731
+    keep = [np.ascontiguousarray(x, dtype=np.uint64) for x in (c_off, cl, a_off, al, b_off, bl)]
+    m = ctypes.c_double()
+    check(lib().bcg_mul_batch_dev(dC.ptr, dA.ptr, dB.ptr, *[k.ctypes.data_as(ctypes.POINTER(_sz)) for k in keep],
+                                  count, None, ctypes.byref(m)), "bcg_mul_batch_dev")
+    C = dC.to_host()
+    out = [C[int(o):int(o) + int(n)].copy() for o, n in zip(c_off, cl)]
+    return out, m.value

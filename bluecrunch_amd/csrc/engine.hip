@@ -43,7 +43,7 @@ struct Engine {
   hipStream_t stream = nullptr;
   std::map<unsigned long long, RootsDev> roots;  // keyed by N
   std::map<int, cd *> ltab;                      // e^{2 pi i t/L}, keyed by L
-  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf;
+  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, margins, counters;
   bool prof = false;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> free_events;
@@ -675,6 +675,92 @@ int fill_zero(uint32_t *C, size_t CL, hipStream_t s) {
   return 0;
 }
 
+typedef void (*mul1_kernel)(ClassArgs);
+
+template <int L>
+int launch_mul1(int grid, hipStream_t s, const ClassArgs &ca) {
+  LAUNCH((k_mul1<L>), grid, (Tile<L, 1>::T), s, ca);
+  return 0;
+}
+
+typedef int (*mul1_fn)(int, hipStream_t, const ClassArgs &);
+
+mul1_fn pick_mul1(int L) {
+  switch (L) {
+    case 64: return &launch_mul1<64>;
+    case 128: return &launch_mul1<128>;
+    case 256: return &launch_mul1<256>;
+    case 512: return &launch_mul1<512>;
+    case 1024: return &launch_mul1<1024>;
+    case 2048: return &launch_mul1<2048>;
+    case 4096: return &launch_mul1<4096>;
+    case 8192: return &launch_mul1<8192>;
+    case 96: return &launch_mul1<96>;
+    case 192: return &launch_mul1<192>;
+    case 384: return &launch_mul1<384>;
+    case 768: return &launch_mul1<768>;
+    case 1536: return &launch_mul1<1536>;
+    case 3072: return &launch_mul1<3072>;
+    case 6144: return &launch_mul1<6144>;
+  }
+  return nullptr;
+}
+
+// Workgroups resident per CU for a one-pass class kernel (LDS bound).
+int mul1_grid(int L, int count) {
+  size_t lds = (size_t)(L + L / 16) * 16 + 256;
+  int per_cu = (int)(160 * 1024 / lds);
+  if (per_cu < 1) per_cu = 1;
+  if (per_cu > 8) per_cu = 8;
+  int grid = 256 * per_cu;
+  return count < grid ? count : grid;
+}
+
+__global__ void k_zero_i32(int *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Launch the one-pass products desc[first .. first+count) of length class L
+// (descriptors and margin slots already on the device).
+int run_class(int L, const MulDesc *ddesc, int count, unsigned int *dmargins, int *dcounter, hipStream_t s) {
+  ClassArgs ca;
+  ca.desc = ddesc;
+  ca.count = count;
+  if (int rc = get_roots((unsigned long long)L, &ca.roots)) return rc;
+  if (int rc = get_ltab(L, &ca.tl)) return rc;
+  ca.margins = dmargins;
+  ca.next = dcounter;
+  mul1_fn f = pick_mul1(L);
+  if (!f) return fail(BCG_E_ARG, "no one-pass kernel for this length");
+  ProfScope ps(BCG_K_MID, s);
+  f(mul1_grid(L, count), s, ca);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// One product whose plan is a single pass: the whole product in one workgroup.
+int mul_onepass(const Plan &p, uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                hipStream_t s, double *margin) {
+  if (int rc = ensure(g.desc, sizeof(MulDesc))) return rc;
+  if (int rc = ensure(g.counters, 64)) return rc;
+  MulDesc d;
+  d.a = A;
+  d.b = B;
+  d.c = C;
+  d.al = AL;
+  d.bl = BL;
+  d.cl = CL;
+  d.bits = p.bits;
+  d.cbits = coef_bits(p);
+  HIPCHK(hipMemcpyAsync(g.desc.p, &d, sizeof d, hipMemcpyHostToDevice, s));
+  if (int rc = zero_margin(s)) return rc;
+  LAUNCH(k_zero_i32, 1, 64, s, (int *)g.counters.p, 1);
+  if (int rc = run_class(p.L[0], (const MulDesc *)g.desc.p, 1, (unsigned int *)g.margin.p, (int *)g.counters.p, s))
+    return rc;
+  return read_margin(s, margin);
+}
+
 int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
                    int force_bits, hipStream_t s, bcg_mul_info *info) {
   bcg_mul_info inf;
@@ -693,6 +779,31 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
   if (int rc = make_plan(AL, BL, force_bits, p)) return rc;
   bool alias = (C == A || C == B);
   for (;;) {
+    if (p.P == 1) {
+      // whole product in one workgroup; aliasing outputs go through a scratch so a
+      // margin retry still sees the inputs
+      double m = 0;
+      uint32_t *dst = C;
+      if (alias) {
+        if (int rc = ensure(g.zbuf, CL * sizeof(uint32_t) + 64)) return rc;
+        dst = (uint32_t *)g.zbuf.p;
+      }
+      if (int rc = mul_onepass(p, dst, CL, A, AL, B, BL, s, &m)) return rc;
+      inf.bits = p.bits;
+      inf.passes = 1;
+      inf.points = (size_t)p.N;
+      inf.margin = m;
+      if (m >= 0.25 && p.bits > 4) {
+        Plan q;
+        if (int rc = make_plan(AL, BL, p.bits - 1, q)) return rc;
+        p = q;
+        inf.retries++;
+        continue;
+      }
+      if (m >= 0.25) return fail(BCG_E_MARGIN, "rounding margin >= 0.25 at the smallest digit size");
+      if (alias) HIPCHK(hipMemcpyAsync(C, dst, CL * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+      break;
+    }
     if (int rc = run_transforms(p, A, AL, B, BL, square, s)) return rc;
     double m = 0;
     if (alias) {
@@ -968,7 +1079,7 @@ int bcg_shutdown(void) {
   std::lock_guard<std::recursive_mutex> lk(g_lock);
   if (!g.ready) return 0;
   (void)hipStreamSynchronize(g.stream);
-  for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf}) {
+  for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf, &g.desc, &g.margins, &g.counters}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->bytes = 0;
@@ -1093,11 +1204,85 @@ int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const s
   ENTER();
   hipStream_t s = pick(stream);
   double worst = 0;
+  // plan every product; one-pass plans are grouped by length class and run as
+  // persistent class kernels, everything else goes through the single path
+  std::map<int, std::vector<size_t>> classes;
+  std::vector<Plan> plans(count);
+  std::vector<size_t> single;
   for (size_t i = 0; i < count; i++) {
+    const uint32_t *pa = A + a_off[i];
+    const uint32_t *pb = B + b_off[i];
+    if (al[i] == 0 || bl[i] == 0 || cl[i] == 0 || al[i] + bl[i] < 100) {
+      single.push_back(i);
+      continue;
+    }
+    if (int rc = make_plan(al[i], bl[i], 0, plans[i])) return rc;
+    bool alias = (C + c_off[i] == pa || C + c_off[i] == pb);
+    if (plans[i].P == 1 && !alias)
+      classes[plans[i].L[0]].push_back(i);
+    else
+      single.push_back(i);
+  }
+  size_t nclass = 0;
+  for (auto &kv : classes) nclass += kv.second.size();
+  if (nclass) {
+    if (int rc = ensure(g.desc, nclass * sizeof(MulDesc))) return rc;
+    if (int rc = ensure(g.margins, nclass * sizeof(unsigned int))) return rc;
+    if (int rc = ensure(g.counters, (classes.size() + 1) * sizeof(int))) return rc;
+    std::vector<MulDesc> hd;
+    std::vector<size_t> order;
+    hd.reserve(nclass);
+    for (auto &kv : classes)
+      for (size_t i : kv.second) {
+        MulDesc d;
+        d.a = A + a_off[i];
+        d.b = B + b_off[i];
+        d.c = C + c_off[i];
+        d.al = al[i];
+        d.bl = bl[i];
+        d.cl = cl[i];
+        d.bits = plans[i].bits;
+        d.cbits = coef_bits(plans[i]);
+        hd.push_back(d);
+        order.push_back(i);
+      }
+    HIPCHK(hipMemcpyAsync(g.desc.p, hd.data(), nclass * sizeof(MulDesc), hipMemcpyHostToDevice, s));
+    LAUNCH(k_zero_u32, (int)((nclass + 255) / 256), 256, s, (unsigned int *)g.margins.p, (int)nclass);
+    LAUNCH(k_zero_i32, 1, 64, s, (int *)g.counters.p, (int)classes.size());
+    size_t first = 0;
+    int ci = 0;
+    for (auto &kv : classes) {
+      int n = (int)kv.second.size();
+      if (int rc = run_class(kv.first, (const MulDesc *)g.desc.p + first, n, (unsigned int *)g.margins.p + first,
+                             (int *)g.counters.p + ci, s))
+        return rc;
+      first += (size_t)n;
+      ci++;
+    }
+    std::vector<unsigned int> mb(nclass);
+    HIPCHK(hipMemcpyAsync(mb.data(), g.margins.p, nclass * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (size_t k = 0; k < nclass; k++) {
+      float f;
+      memcpy(&f, &mb[k], sizeof f);
+      if (f >= 0.25f) {
+        single.push_back(order[k]);  // redo with a smaller digit size
+        continue;
+      }
+      if (f > worst) worst = f;
+    }
+  }
+  for (size_t i : single) {
     bcg_mul_info inf;
     const uint32_t *pa = A + a_off[i];
     const uint32_t *pb = B + b_off[i];
-    if (int rc = mul_dev_locked(C + c_off[i], cl[i], pa, al[i], pb, bl[i], 0, s, &inf)) return rc;
+    bool forced = false;
+    int fb = 0;
+    if (plans[i].N && plans[i].P == 1 && !(C + c_off[i] == pa || C + c_off[i] == pb)) {
+      fb = plans[i].bits - 1;  // came back from a class kernel with margin >= 0.25
+      forced = true;
+    }
+    if (int rc = mul_dev_locked(C + c_off[i], cl[i], pa, al[i], pb, bl[i], forced ? fb : 0, s, &inf)) return rc;
     if (inf.margin > worst) worst = inf.margin;
   }
   if (worst_margin) *worst_margin = worst;

@@ -257,15 +257,13 @@ __device__ __forceinline__ void mid_forward(const PassArgs &a, cd *lds, const cd
 }
 
 // Last forward pass of A (and B), pointwise product (or square), first inverse
-// pass, all on one contiguous block of L points. With PACK/UNPACK the block is
-// the whole transform (one-pass plans).
-template <int L, bool SQUARE, bool PACK, bool UNPACK>
-__global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
+// pass on one contiguous block of L points. Ends with the outermost inverse
+// stage's outputs in registers: out[i * R0 + t] holds row q + t * (L / R0),
+// q = tid + i * T (natural order). The caller stores or unpacks them.
+template <int L, bool SQUARE, bool PACK>
+__device__ __forceinline__ void mid_pipeline(const PassArgs &a, cd *lds, unsigned long long blk0,
+                                             cd (&xa)[Tile<L, 1>::EPT], int tid) {
   using TC = Tile<L, 1>;
-  __shared__ cd lds[TC::LDS_ELEMS];
-  const int tid = threadIdx.x;
-  const unsigned long long blk0 = (unsigned long long)blockIdx.x * L;
-  cd xa[TC::EPT];
   mid_forward<L, PACK>(a, lds, a.data, a.wa, a.al, blk0, xa, tid);
   if (!SQUARE) {
     cd xb[TC::EPT];
@@ -277,7 +275,6 @@ __global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
 #pragma unroll
     for (int i = 0; i < TC::EPT; i++) xa[i] = cmul(xa[i], xa[i]);
   }
-  // Inverse: innermost stage in registers, mirror of mid_forward.
   constexpr int RB0 = TC::FIRSTB;
   constexpr int R0 = 1 << RB0;
   constexpr int PER0 = TC::EPT / R0;
@@ -322,7 +319,6 @@ __global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
       __syncthreads();
     }
   }
-  double worst = 0.0;
 #pragma unroll
   for (int i = 0; i < PER0; i++) {
     int q = tid + i * TC::T;
@@ -336,12 +332,34 @@ __global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
     }
     dit_levels<RB0>(x, a.tl, q, 1);
 #pragma unroll
+    for (int t = 0; t < R0; t++) xa[i * R0 + t] = x[t];
+  }
+}
+
+// Middle pass of a multi-pass plan (in place into a.data), or a whole one-pass
+// product rounded into a.coef when PACK/UNPACK.
+template <int L, bool SQUARE, bool PACK, bool UNPACK>
+__global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
+  using TC = Tile<L, 1>;
+  __shared__ cd lds[TC::LDS_ELEMS];
+  const int tid = threadIdx.x;
+  const unsigned long long blk0 = (unsigned long long)blockIdx.x * L;
+  cd xa[TC::EPT];
+  mid_pipeline<L, SQUARE, PACK>(a, lds, blk0, xa, tid);
+  constexpr int R0 = 1 << TC::FIRSTB;
+  constexpr int PER0 = TC::EPT / R0;
+  constexpr int SUB0 = L / R0;
+  double worst = 0.0;
+#pragma unroll
+  for (int i = 0; i < PER0; i++) {
+    int q = tid + i * TC::T;
+#pragma unroll
     for (int t = 0; t < R0; t++) {
       unsigned long long j = blk0 + (unsigned long long)(q + t * SUB0);
       if (UNPACK)
-        worst = fmax(worst, unpack_point(a, x[t], j));
+        worst = fmax(worst, unpack_point(a, xa[i * R0 + t], j));
       else
-        a.data[j] = x[t];
+        a.data[j] = xa[i * R0 + t];
     }
   }
   if (UNPACK) margin_reduce(a.margin, worst);
@@ -495,6 +513,175 @@ __global__ void k_fill_u32(uint32_t *p, unsigned long long n, uint32_t v) {
   unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
   for (; i < n; i += st) p[i] = v;
+}
+
+}  // namespace bcg
+
+namespace bcg {
+
+// ------------------------------------------- one-workgroup products (batched)
+
+// Carry functions of single-bit carry chains: bit0 = carry out for carry in 0,
+// bit1 = carry out for carry in 1. compose(f, g) applies f first, then g.
+__device__ __forceinline__ unsigned carry_compose(unsigned f, unsigned g) {
+  unsigned h0 = (f & 1u) ? (g >> 1) & 1u : g & 1u;
+  unsigned h1 = (f & 2u) ? (g >> 1) & 1u : g & 1u;
+  return h0 | (h1 << 1);
+}
+
+// P_w for one word from coefficients held in LDS (see word_partials).
+__device__ __forceinline__ unsigned long long word_partial_lds(const long long *coef, long long ncoef, int bits,
+                                                               int cbits, long long w) {
+  if (w < 0) return 0ull;
+  long long lo_bit = 32 * w - cbits + 1;
+  long long jlo = lo_bit <= 0 ? 0 : (lo_bit + bits - 1) / bits;
+  long long jhi = (32 * w + 31) / bits;
+  if (jhi >= ncoef) jhi = ncoef - 1;
+  unsigned long long P = 0;
+  for (long long j = jlo; j <= jhi; j++) {
+    unsigned long long c = (unsigned long long)coef[j];
+    long long sh = (long long)bits * j - 32 * w;
+    if (sh >= 0)
+      P += (unsigned long long)(uint32_t)(c << sh);
+    else if (-sh < 64)
+      P += (unsigned long long)(uint32_t)(c >> (-sh));
+  }
+  return P;
+}
+
+// Words out[0..CL) = (sum_j coef[j] 2^{bits j}) mod 2^{32 CL}, coefficients in
+// LDS, carries resolved with a workgroup scan of carry functions.
+template <int T>
+__device__ void wg_carry(const long long *coef, long long ncoef, int bits, int cbits, uint32_t *out, long long CL,
+                         int tid) {
+  __shared__ unsigned wave_fn[T / 64 + 1];
+  const long long per = (CL + T - 1) / T;
+  const long long w0 = (long long)tid * per;
+  long long w1 = w0 + per;
+  if (w1 > CL) w1 = CL;
+  // this thread's carry function over words [w0, w1)
+  unsigned f = 2u;  // identity
+  {
+    unsigned long long prev = word_partial_lds(coef, ncoef, bits, cbits, w0 - 1);
+    unsigned c0 = 0, c1 = 1;
+    for (long long w = w0; w < w1; w++) {
+      unsigned long long P = word_partial_lds(coef, ncoef, bits, cbits, w);
+      unsigned long long v = (P & 0xffffffffull) + (prev >> 32);
+      c0 = (unsigned)((v + c0) >> 32);
+      c1 = (unsigned)((v + c1) >> 32);
+      prev = P;
+    }
+    if (w0 < w1) f = c0 | (c1 << 1);
+  }
+  // inclusive scan within the wave
+  const int lane = tid & 63, wave = tid >> 6;
+  unsigned inc = f;
+  for (int off = 1; off < 64; off <<= 1) {
+    unsigned g = __shfl_up(inc, off, 64);
+    if (lane >= off) inc = carry_compose(g, inc);
+  }
+  unsigned excl = __shfl_up(inc, 1, 64);
+  if (lane == 0) excl = 2u;
+  if (lane == 63) wave_fn[wave] = inc;
+  __syncthreads();
+  unsigned pre = 2u;
+  for (int w = 0; w < wave; w++) pre = carry_compose(pre, wave_fn[w]);
+  unsigned cin = carry_compose(pre, excl) & 1u;  // carry into word w0 (block carry-in is 0)
+  // write the words
+  unsigned long long prev = word_partial_lds(coef, ncoef, bits, cbits, w0 - 1);
+  for (long long w = w0; w < w1; w++) {
+    unsigned long long P = word_partial_lds(coef, ncoef, bits, cbits, w);
+    unsigned long long v = (P & 0xffffffffull) + (prev >> 32) + cin;
+    out[w] = (uint32_t)v;
+    cin = (unsigned)(v >> 32);
+    prev = P;
+  }
+}
+
+// One product of a batch.
+struct MulDesc {
+  const uint32_t *a;
+  const uint32_t *b;
+  uint32_t *c;
+  unsigned long long al, bl, cl;
+  int bits;
+  int cbits;
+};
+
+// All products of one transform length class L (one-pass plans: N = L).
+struct ClassArgs {
+  const MulDesc *desc;
+  int count;
+  Roots roots;
+  const cd *tl;
+  unsigned int *margins;  // per product, float bits of the worst rounding distance
+  int *next;              // work counter, zero at launch
+};
+
+// Persistent workgroups pull products of one length class from a counter and
+// run each product whole: pack, forward, pointwise, inverse, rounding and the
+// carry resolve, with the coefficients never leaving LDS.
+template <int L>
+__global__ __launch_bounds__((Tile<L, 1>::T)) void k_mul1(ClassArgs ca) {
+  using TC = Tile<L, 1>;
+  __shared__ cd lds[TC::LDS_ELEMS];
+  __shared__ int job;
+  const int tid = threadIdx.x;
+  constexpr int R0 = 1 << TC::FIRSTB;
+  constexpr int PER0 = TC::EPT / R0;
+  constexpr int SUB0 = L / R0;
+  for (;;) {
+    if (tid == 0) job = atomicAdd(ca.next, 1);
+    __syncthreads();
+    const int idx = job;
+    __syncthreads();
+    if (idx >= ca.count) break;
+    const MulDesc d = ca.desc[idx];
+    PassArgs a;
+    a.data = nullptr;
+    a.data_b = nullptr;
+    a.wa = d.a;
+    a.wb = d.b;
+    a.al = d.al;
+    a.bl = d.bl;
+    a.coef = nullptr;
+    a.N = L;
+    a.M = 1;
+    a.S = L;
+    a.tw_mul = 4;
+    a.tl = ca.tl;
+    a.roots = ca.roots;
+    a.bits = d.bits;
+    a.ntiles = 1;
+    a.margin = nullptr;
+    a.inv_n = 1.0 / (double)L;
+    cd xa[TC::EPT];
+    if (d.a == d.b && d.al == d.bl)
+      mid_pipeline<L, true, true>(a, lds, 0, xa, tid);
+    else
+      mid_pipeline<L, false, true>(a, lds, 0, xa, tid);
+    __syncthreads();  // LDS now becomes the coefficient array
+    long long *coef = (long long *)lds;
+    double worst = 0.0;
+#pragma unroll
+    for (int i = 0; i < PER0; i++) {
+      int q = tid + i * TC::T;
+#pragma unroll
+      for (int t = 0; t < R0; t++) {
+        int j = q + t * SUB0;
+        cd f = cmulc(xa[i * R0 + t], omega_pow(a.roots, (unsigned long long)j));
+        double re = f.x * a.inv_n, im = f.y * a.inv_n;
+        worst = fmax(worst, fmax(fabs(re - rint(re)), fabs(im - rint(im))));
+        long long cr = (long long)(re + 0.5), ci = (long long)(im + 0.5);
+        coef[j] = cr > 0 ? cr : 0;
+        coef[j + L] = ci > 0 ? ci : 0;
+      }
+    }
+    margin_reduce(ca.margins + idx, worst);
+    __syncthreads();
+    wg_carry<TC::T>(coef, 2 * L, d.bits, d.cbits, d.c, (long long)d.cl, tid);
+    __syncthreads();
+  }
 }
 
 }  // namespace bcg

@@ -1,0 +1,44 @@
+"""Batched uneven products (persistent one-workgroup class kernels) against
+exact integer products."""
+import numpy as np
+import pytest
+
+from helpers import exact_product
+from bluecrunch_amd.workload import SplitMix64, batch_lengths, operand
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(seed, count, lo=16, hi=16384):
+    lens = batch_lengths(seed, count, lo, hi)
+    rng = SplitMix64(seed + 1)
+    return [(operand(rng, int(x)), operand(rng, int(y))) for x, y in lens]
+
+
+def test_batch_small_uneven(bc):
+    pairs = make_batch(7, 200, 16, 4096)
+    out, worst = bc.mul_batch(pairs)
+    for (a, b), c in zip(pairs, out):
+        assert np.array_equal(c, exact_product(a, b))
+    assert worst < 0.25
+
+
+def test_batch_mixed_with_squares_and_tiny(bc):
+    rng = SplitMix64(99)
+    pairs = []
+    for n in [1, 2, 10, 49, 50, 60, 200, 1000, 3000, 8000, 20000]:
+        a = operand(rng, n)
+        pairs.append((a, a))                  # square (same array object)
+        pairs.append((a, operand(rng, max(1, n // 3))))
+    out, worst = bc.mul_batch(pairs)
+    for (a, b), c in zip(pairs, out):
+        assert np.array_equal(c, exact_product(a, b))
+
+
+def test_batch_benchmark_shape(bc):
+    # the batched workload: log-uniform operand lengths in [16, 16384], seed 7
+    pairs = make_batch(7, 512)
+    out, worst = bc.mul_batch(pairs)
+    bad = [i for i, ((a, b), c) in enumerate(zip(pairs, out)) if not np.array_equal(c, exact_product(a, b))]
+    assert not bad
+    assert worst < 0.25

@@ -28,8 +28,8 @@ EXPORTS = {
         "bcg_mul_batch_dev", "bcg_fft_forward_dev", "bcg_fft_inverse_dev", "bcg_fft_pointwise_dev",
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
-        "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_prof_enable", "bcg_prof_read",
-        "bcg_prof_reset",
+        "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
+        "bcg_plan_force", "bcg_autotune", "bcg_prof_enable", "bcg_prof_read", "bcg_prof_reset",
     ],
     "fft.h": [
         "twiddle_table", "twiddle_table_size", "fft_ensure_table", "table_select", "fft_free_table",
@@ -104,6 +104,9 @@ def lib():
     L.bcg_tft_forward_host.argtypes = [_f64p, _sz, i]
     L.bcg_tft_inverse_host.argtypes = [_f64p, _sz, i]
     L.bcg_plan_query.argtypes = [_sz, _sz, ctypes.POINTER(i), ctypes.POINTER(_sz), ctypes.POINTER(i)]
+    L.bcg_plan_candidates.argtypes = [_sz, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.bcg_plan_force.argtypes = [_sz, i]
+    L.bcg_autotune.argtypes = [_sz, _sz, i, i, ctypes.POINTER(i), _f64p, i]
     L.bcg_prof_enable.argtypes = [i]
     L.bcg_prof_read.argtypes = [i, ctypes.POINTER(ctypes.c_long), _f64p]
     # reference C API
@@ -336,6 +339,35 @@ def mul_dev(c, cl, a, al, b, bl, bits=0, stream=None):
     info = MulInfo()
     check(lib().bcg_mul_dev_bits(c, cl, a, al, b, bl, bits, stream, ctypes.byref(info)), "bcg_mul_dev")
     return info.as_dict()
+
+
+def plan_candidates(n):
+    """Pass splits available for an n-point transform: list of (passes, L, C)."""
+    L = lib()
+    count = L.bcg_plan_candidates(n, -1, None, None, None)
+    if count < 0:
+        check(count, "plan_candidates")
+    out = []
+    for i in range(count):
+        P = ctypes.c_int()
+        Ls = (ctypes.c_int * 3)()
+        Cs = (ctypes.c_int * 3)()
+        L.bcg_plan_candidates(n, i, ctypes.byref(P), Ls, Cs)
+        out.append((P.value, list(Ls)[:P.value], list(Cs)[:P.value]))
+    return out
+
+
+def plan_force(n, index):
+    check(lib().bcg_plan_force(n, index), "plan_force")
+
+
+def autotune(al, bl, square=False, reps=2):
+    """Time every pass split for an al x bl product; returns (chosen, times_ms)."""
+    n = len(plan_candidates(plan(al, bl)["points"]))
+    times = (ctypes.c_double * max(n, 1))()
+    chosen = ctypes.c_int()
+    check(lib().bcg_autotune(al, bl, 1 if square else 0, reps, ctypes.byref(chosen), times, n), "autotune")
+    return chosen.value, list(times)[:n]
 
 
 def prof_enable(on=True):

@@ -289,41 +289,80 @@ int tile_cols(int L, unsigned long long M) {
   return c;
 }
 
-// Split N into passes: innermost contiguous pass up to 8192 (6144 for 3*2^k),
-// outer passes >= 64 rows.
-bool factor(unsigned long long N, Plan &p) {
-  const unsigned long long MAXIN = is_three(N) ? 6144 : 8192;
-  if (N <= MAXIN) {
-    p.P = 1;
-    p.L[0] = (int)N;
-    return true;
-  }
-  unsigned long long p2 = is_three(N) ? N / 3 : N;  // power of two part
-  unsigned long long inner = MAXIN;
-  if (N / inner < 64) inner = N / 64;  // keep the outer pass at least 64 long
-  unsigned long long outer = N / inner;
-  (void)p2;
-  if (outer <= 8192) {
-    p.P = 2;
-    p.L[0] = (int)outer;
-    p.L[1] = (int)inner;
-  } else {
-    unsigned long long rest = outer;  // power of two, split into two passes
-    int lg = ilog2u(rest);
-    unsigned long long l1 = 1ull << ((lg + 1) / 2);
-    unsigned long long l2 = rest / l1;
-    if (l1 > 8192 || l2 > 8192 || l2 < 64) return false;
-    p.P = 3;
-    p.L[0] = (int)l1;
-    p.L[1] = (int)l2;
-    p.L[2] = (int)inner;
-  }
-  unsigned long long m = N;
+bool fill_strides(Plan &p) {
+  unsigned long long prod = 1;
+  for (int i = 0; i < p.P; i++) prod *= (unsigned long long)p.L[i];
+  if (prod != p.N) return false;
+  unsigned long long m = p.N;
   for (int i = 0; i < p.P; i++) {
     m /= (unsigned long long)p.L[i];
     p.M[i] = m;
     p.C[i] = (i == p.P - 1) ? 1 : tile_cols(p.L[i], m);
   }
+  return true;
+}
+
+bool is_pow2(unsigned long long v) { return v && !(v & (v - 1)); }
+
+// Every pass split of N the kernels support: the innermost (contiguous) pass
+// carries the 3*2^k factor and is <= 8192 (<= 6144); outer passes are powers
+// of two in [64, 8192]; at most three passes. Candidate 0 is the default:
+// the largest inner pass, then the fewest passes.
+std::vector<Plan> factor_candidates(unsigned long long N) {
+  std::vector<Plan> out;
+  const bool three = is_three(N);
+  const unsigned long long MAXIN = three ? 6144 : 8192;
+  if (N <= MAXIN) {
+    Plan p;
+    p.N = N;
+    p.P = 1;
+    p.L[0] = (int)N;
+    fill_strides(p);
+    out.push_back(p);
+    return out;
+  }
+  for (unsigned long long inner = MAXIN; inner >= (three ? 192ull : 128ull); inner /= 2) {
+    if (N % inner) continue;
+    unsigned long long rest = N / inner;
+    if (!is_pow2(rest)) continue;
+    if (rest >= 64 && rest <= 8192) {
+      Plan p;
+      p.N = N;
+      p.P = 2;
+      p.L[0] = (int)rest;
+      p.L[1] = (int)inner;
+      if (fill_strides(p)) out.push_back(p);
+    }
+    for (unsigned long long l1 = 64; l1 <= 8192; l1 *= 2) {
+      unsigned long long l2 = rest / l1;
+      if (l1 * l2 != rest || l2 < 64 || l2 > 8192) continue;
+      Plan p;
+      p.N = N;
+      p.P = 3;
+      p.L[0] = (int)l1;
+      p.L[1] = (int)l2;
+      p.L[2] = (int)inner;
+      if (fill_strides(p)) out.push_back(p);
+    }
+  }
+  return out;
+}
+
+// Tuned choice per transform length: index into factor_candidates(N).
+std::map<unsigned long long, int> g_tuned;
+
+bool factor(unsigned long long N, Plan &p) {
+  std::vector<Plan> c = factor_candidates(N);
+  if (c.empty()) return false;
+  int idx = 0;
+  auto it = g_tuned.find(N);
+  if (it != g_tuned.end() && it->second >= 0 && it->second < (int)c.size()) idx = it->second;
+  int bits = p.bits;
+  unsigned long long dA = p.dA, dB = p.dB;
+  p = c[idx];
+  p.bits = bits;
+  p.dA = dA;
+  p.dB = dB;
   return true;
 }
 
@@ -1064,6 +1103,18 @@ int tft_inverse_rec(cd *y, unsigned long long len, unsigned long long n, cd *x, 
   return rc;
 }
 
+__global__ void k_fill_hash(uint32_t *p, unsigned long long n, unsigned long long seed) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    unsigned long long z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    p[i] = (uint32_t)((z >> 32) % 1000000000ull) | (i + 1 == n ? 1u : 0u);
+  }
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1460,6 +1511,88 @@ int bcg_plan_query(size_t AL, size_t BL, int *bits, size_t *points, int *passes)
   if (bits) *bits = p.bits;
   if (points) *points = (size_t)p.N;
   if (passes) *passes = p.P;
+  return 0;
+}
+
+
+int bcg_plan_candidates(size_t N, int index, int *passes, int *L, int *C) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  if (!valid_length(N) || N < 64) return fail(BCG_E_ARG, "length must be 2^k or 3*2^k, >= 64");
+  std::vector<Plan> c = factor_candidates(N);
+  if (index < 0) return (int)c.size();
+  if (index >= (int)c.size()) return fail(BCG_E_ARG, "candidate index out of range");
+  const Plan &p = c[index];
+  if (passes) *passes = p.P;
+  for (int i = 0; i < 3; i++) {
+    if (L) L[i] = i < p.P ? p.L[i] : 0;
+    if (C) C[i] = i < p.P ? p.C[i] : 0;
+  }
+  return (int)c.size();
+}
+
+int bcg_plan_force(size_t N, int index) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  if (index < 0) {
+    g_tuned.erase(N);
+    return 0;
+  }
+  if (index >= (int)factor_candidates(N).size()) return fail(BCG_E_ARG, "candidate index out of range");
+  g_tuned[N] = index;
+  return 0;
+}
+
+int bcg_autotune(size_t AL, size_t BL, int square, int reps, int *chosen, double *times_ms, int max_out) {
+  ENTER();
+  if (AL == 0 || BL == 0 || AL + BL < 100) return fail(BCG_E_ARG, "autotune needs an FFT-sized product");
+  if (square) BL = AL;
+  if (reps < 1) reps = 1;
+  Plan p0;
+  if (int rc = make_plan(AL, BL, 0, p0)) return rc;
+  const unsigned long long N = p0.N;
+  std::vector<Plan> cands = factor_candidates(N);
+  hipStream_t s = g.stream;
+  uint32_t *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  HIPCHK(hipMalloc(&dA, AL * 4));
+  HIPCHK(hipMalloc(&dB, BL * 4));
+  HIPCHK(hipMalloc(&dC, (AL + BL) * 4));
+  LAUNCH(k_fill_hash, grid_for(AL), 256, s, dA, (unsigned long long)AL, 1ull);
+  LAUNCH(k_fill_hash, grid_for(BL), 256, s, dB, (unsigned long long)BL, 2ull);
+  const uint32_t *pB = square ? dA : dB;
+  hipEvent_t e0 = get_event(), e1 = get_event();
+  int best = -1;
+  double best_ms = 0;
+  int rc = 0;
+  auto saved = g_tuned.find(N) != g_tuned.end() ? g_tuned[N] : -1;
+  for (int i = 0; i < (int)cands.size() && !rc; i++) {
+    g_tuned[N] = i;
+    bcg_mul_info inf;
+    rc = mul_dev_locked(dC, AL + BL, dA, AL, pB, BL, 0, s, &inf);  // warm-up (tables, buffers)
+    if (rc) break;
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps && !rc; r++) rc = mul_dev_locked(dC, AL + BL, dA, AL, pB, BL, 0, s, &inf);
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    double per = ms / reps;
+    if (times_ms && i < max_out) times_ms[i] = per;
+    if (best < 0 || per < best_ms) {
+      best = i;
+      best_ms = per;
+    }
+  }
+  if (rc || best < 0) {
+    if (saved >= 0) g_tuned[N] = saved; else g_tuned.erase(N);
+  } else {
+    g_tuned[N] = best;
+  }
+  g.free_events.push_back(e0);
+  g.free_events.push_back(e1);
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dC);
+  if (rc) return rc;
+  if (chosen) *chosen = best;
   return 0;
 }
 

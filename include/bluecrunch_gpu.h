@@ -101,6 +101,18 @@ int bcg_tft_inverse_host(double *z, size_t len, int k);
 /* Plan inspection: the digit size and transform size a product would use. */
 int bcg_plan_query(size_t AL, size_t BL, int *bits, size_t *points, int *passes);
 
+/* Pass splits of a transform of N points (N = 2^k or 3*2^k, >= 64). Returns
+ * the number of candidates; with index >= 0 also the split of that one
+ * (passes, L[3] pass lengths, C[3] columns per workgroup). */
+int bcg_plan_candidates(size_t N, int index, int *passes, int *L, int *C);
+/* Use candidate `index` for every transform of N points (-1: back to the
+ * default or the autotuned choice). */
+int bcg_plan_force(size_t N, int index);
+/* Autotuner: times every pass split of the transform an AL x BL product uses
+ * (synthetic words, `reps` runs each) and keeps the fastest for that length.
+ * times_ms (optional) receives up to max_out per-candidate milliseconds. */
+int bcg_autotune(size_t AL, size_t BL, int square, int reps, int *chosen, double *times_ms, int max_out);
+
 /* Kernel timing with HIP events recorded on the launch stream. While enabled,
  * every kernel the engine launches is bracketed by events; bcg_prof_read
  * returns, per kernel kind, launch count and summed milliseconds. */

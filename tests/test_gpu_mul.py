@@ -172,3 +172,30 @@ def test_forced_digit_sizes(bc, bits):
     info = bc.mul_dev(dC.ptr, 40000, dA.ptr, 20000, dB.ptr, 20000, bits=bits)
     assert info["bits"] <= bits
     assert np.array_equal(dC.to_host(), exact_product(a, b))
+
+
+@pytest.mark.parametrize("words", [9000, 100000, 700000])
+def test_every_pass_split_is_exact(bc, words):
+    # every factorisation the autotuner may pick must give the same exact product
+    a, b = operands(words, words, words)
+    want = exact_product(a, b)
+    n = bc.plan(words, words)["points"]
+    cands = bc.plan_candidates(n)
+    assert cands
+    try:
+        for i in range(len(cands)):
+            bc.plan_force(n, i)
+            c, info = bc.mul(a, b)
+            assert np.array_equal(c, want), (i, cands[i])
+            s, _ = bc.mul(a, a)
+            assert np.array_equal(s, exact_product(a, a)), (i, cands[i])
+    finally:
+        bc.plan_force(n, -1)
+
+
+def test_autotune_keeps_exactness(bc):
+    chosen, times = bc.autotune(300000, 300000, reps=1)
+    assert 0 <= chosen < len(times) and all(t > 0 for t in times)
+    a, b = operands(4, 300000, 300000)
+    c, _ = bc.mul(a, b)
+    assert product_fingerprint_ok(a, b, c)

@@ -43,7 +43,7 @@ struct Engine {
   hipStream_t stream = nullptr;
   std::map<unsigned long long, RootsDev> roots;  // keyed by N
   std::map<int, cd *> ltab;                      // e^{2 pi i t/L}, keyed by L
-  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, margins, counters;
+  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, margins, counters, bbufA, bbufB, bblk, bdesc;
   bool prof = false;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> free_events;
@@ -393,37 +393,41 @@ int make_plan(unsigned long long AL, unsigned long long BL, int force_bits, Plan
 #define LAUNCH(kern, grid, block, stream, ...) \
   hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, __VA_ARGS__)
 
+// grid.y > 1 only for batched launches (PassArgs::desc set).
+#define LAUNCH2(kern, gx, gy, block, stream, ...) \
+  hipLaunchKernelGGL(kern, dim3(gx, gy), dim3(block), 0, stream, __VA_ARGS__)
+
 template <int L, int C>
-int launch_fwd(bool pack, int grid, hipStream_t s, const PassArgs &a) {
+int launch_fwd(bool pack, int grid, int gy, hipStream_t s, const PassArgs &a) {
   if (pack)
-    LAUNCH((k_fwd<L, C, true>), grid, (Tile<L, C>::T), s, a);
+    LAUNCH2((k_fwd<L, C, true>), grid, gy, (Tile<L, C>::T), s, a);
   else
-    LAUNCH((k_fwd<L, C, false>), grid, (Tile<L, C>::T), s, a);
+    LAUNCH2((k_fwd<L, C, false>), grid, gy, (Tile<L, C>::T), s, a);
   return 0;
 }
 
 template <int L, int C>
-int launch_inv(bool unpack, int grid, hipStream_t s, const PassArgs &a) {
+int launch_inv(bool unpack, int grid, int gy, hipStream_t s, const PassArgs &a) {
   if (unpack)
-    LAUNCH((k_inv<L, C, true>), grid, (Tile<L, C>::T), s, a);
+    LAUNCH2((k_inv<L, C, true>), grid, gy, (Tile<L, C>::T), s, a);
   else
-    LAUNCH((k_inv<L, C, false>), grid, (Tile<L, C>::T), s, a);
+    LAUNCH2((k_inv<L, C, false>), grid, gy, (Tile<L, C>::T), s, a);
   return 0;
 }
 
 template <int L>
-int launch_mid(bool sq, bool pack, int grid, hipStream_t s, const PassArgs &a) {
+int launch_mid(bool sq, bool pack, int grid, int gy, hipStream_t s, const PassArgs &a) {
   constexpr int T = Tile<L, 1>::T;
   if (pack) {
     if (sq)
-      LAUNCH((k_mid<L, true, true, true>), grid, T, s, a);
+      LAUNCH2((k_mid<L, true, true, true>), grid, gy, T, s, a);
     else
-      LAUNCH((k_mid<L, false, true, true>), grid, T, s, a);
+      LAUNCH2((k_mid<L, false, true, true>), grid, gy, T, s, a);
   } else {
     if (sq)
-      LAUNCH((k_mid<L, true, false, false>), grid, T, s, a);
+      LAUNCH2((k_mid<L, true, false, false>), grid, gy, T, s, a);
     else
-      LAUNCH((k_mid<L, false, false, false>), grid, T, s, a);
+      LAUNCH2((k_mid<L, false, false, false>), grid, gy, T, s, a);
   }
   return 0;
 }
@@ -436,7 +440,7 @@ int launch_mid(bool sq, bool pack, int grid, hipStream_t s, const PassArgs &a) {
   case 8: return X<L, 8>;     \
   case 16: return X<L, 16>;
 
-typedef int (*fwd_fn)(bool, int, hipStream_t, const PassArgs &);
+typedef int (*fwd_fn)(bool, int, int, hipStream_t, const PassArgs &);
 
 template <int L>
 fwd_fn pick_fwd_c(int C) {
@@ -504,7 +508,7 @@ fwd_fn pick_inv(int L, int C) {
   return nullptr;
 }
 
-typedef int (*mid_fn)(bool, bool, int, hipStream_t, const PassArgs &);
+typedef int (*mid_fn)(bool, bool, int, int, hipStream_t, const PassArgs &);
 
 mid_fn pick_mid(int L) {
   switch (L) {
@@ -555,6 +559,7 @@ int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbi
   unsigned long long nblk = (CL + wpb - 1) / wpb;
   if (int rc = ensure(g.blk, nblk)) return rc;
   CarryArgs c;
+  memset(&c, 0, sizeof c);
   c.coef = coef;
   c.ncoef = ncoef;
   c.bits = bits;
@@ -612,7 +617,7 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     mid_fn f = pick_mid(p.L[0]);
     if (!f) return fail(BCG_E_ARG, "no kernel for this length");
     ProfScope ps(BCG_K_MID, s);
-    f(square, true, 1, s, a);
+    f(square, true, 1, 1, s, a);
     HIPCHK(hipGetLastError());
     return 0;
   }
@@ -628,7 +633,7 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
       fwd_fn f = pick_fwd(p.L[i], p.C[i]);
       if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
       ProfScope ps(i == 0 ? BCG_K_FWD_PACK : BCG_K_FWD, s);
-      f(i == 0, b.ntiles, s, b);
+      f(i == 0, b.ntiles, 1, s, b);
       HIPCHK(hipGetLastError());
     }
   }
@@ -642,7 +647,7 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     mid_fn f = pick_mid(p.L[i]);
     if (!f) return fail(BCG_E_ARG, "no middle kernel for this length");
     ProfScope ps(BCG_K_MID, s);
-    f(square, false, (int)(p.N / (unsigned long long)p.L[i]), s, b);
+    f(square, false, (int)(p.N / (unsigned long long)p.L[i]), 1, s, b);
     HIPCHK(hipGetLastError());
   }
   // inverse outer passes, the outermost one rounding into coefficients
@@ -654,7 +659,7 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     fwd_fn f = pick_inv(p.L[i], p.C[i]);
     if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
     ProfScope ps(i == 0 ? BCG_K_INV_UNPACK : BCG_K_INV, s);
-    f(i == 0, b.ntiles, s, b);
+    f(i == 0, b.ntiles, 1, s, b);
     HIPCHK(hipGetLastError());
   }
   return 0;
@@ -800,6 +805,82 @@ int mul_onepass(const Plan &p, uint32_t *C, size_t CL, const uint32_t *A, size_t
   return read_margin(s, margin);
 }
 
+// Batched multi-pass products of one plan (same N and pass split): every pass
+// is one launch over (tiles x products); product p owns arrays p*N in the
+// batch buffers and margin slot p. Descriptors already on the device.
+int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<MulDesc> &hdesc,
+                        unsigned int *dmargins, hipStream_t s) {
+  const int count = (int)hdesc.size();
+  const size_t arr = (size_t)p.N * sizeof(cd) * (size_t)count;
+  if (int rc = ensure(g.bbufA, arr)) return rc;
+  if (int rc = ensure(g.bbufB, arr)) return rc;
+  cd *da = (cd *)g.bbufA.p;
+  cd *db = (cd *)g.bbufB.p;
+  PassArgs a;
+  memset(&a, 0, sizeof a);
+  a.desc = ddesc;
+  a.margin = dmargins;
+  for (int op = 0; op < 2; op++) {
+    for (int i = 0; i < p.P - 1; i++) {
+      PassArgs b = a;
+      if (int rc = pass_args(p, i, b)) return rc;
+      b.data = op == 0 ? da : db;
+      b.operand = op;
+      fwd_fn f = pick_fwd(p.L[i], p.C[i]);
+      if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
+      ProfScope ps(i == 0 ? BCG_K_FWD_PACK : BCG_K_FWD, s);
+      f(i == 0, b.ntiles, count, s, b);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  {
+    PassArgs b = a;
+    int i = p.P - 1;
+    if (int rc = pass_args(p, i, b)) return rc;
+    b.data = da;
+    b.data_b = db;
+    mid_fn f = pick_mid(p.L[i]);
+    if (!f) return fail(BCG_E_ARG, "no middle kernel for this length");
+    ProfScope ps(BCG_K_MID, s);
+    f(false, false, (int)(p.N / (unsigned long long)p.L[i]), count, s, b);
+    HIPCHK(hipGetLastError());
+  }
+  for (int i = p.P - 2; i >= 0; i--) {
+    PassArgs b = a;
+    if (int rc = pass_args(p, i, b)) return rc;
+    b.data = da;
+    b.coef = (long long *)db;
+    fwd_fn f = pick_inv(p.L[i], p.C[i]);
+    if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
+    ProfScope ps(i == 0 ? BCG_K_INV_UNPACK : BCG_K_INV, s);
+    f(i == 0, b.ntiles, count, s, b);
+    HIPCHK(hipGetLastError());
+  }
+  // carries, one product per grid row
+  const unsigned long long wpb = (unsigned long long)CT * CWPT;
+  unsigned long long nblk_max = 1;
+  for (const MulDesc &d : hdesc) {
+    unsigned long long nb = (d.cl + wpb - 1) / wpb;
+    if (nb > nblk_max) nblk_max = nb;
+  }
+  if (int rc = ensure(g.bblk, nblk_max * (size_t)count)) return rc;
+  CarryArgs c;
+  memset(&c, 0, sizeof c);
+  c.desc = ddesc;
+  c.coef_stride = 2 * p.N;
+  c.coef = (const long long *)db;
+  c.ncoef = 2 * p.N;
+  c.blk_fn = (unsigned char *)g.bblk.p;
+  c.nblk = nblk_max;
+  {
+    ProfScope ps(BCG_K_CARRY, s);
+    LAUNCH2(k_carry_local, (int)nblk_max, count, CT, s, c);
+    LAUNCH(k_carry_fix, count, 1024, s, c);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
                    int force_bits, hipStream_t s, bcg_mul_info *info) {
   bcg_mul_info inf;
@@ -938,7 +1019,7 @@ int transform_dev(double *z, size_t length, bool inverse, hipStream_t s) {
       fwd_fn f = pick_fwd(p.L[i], p.C[i]);
       if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
       ProfScope ps(BCG_K_FWD, s);
-      f(false, b.ntiles, s, b);
+      f(false, b.ntiles, 1, s, b);
       HIPCHK(hipGetLastError());
     }
   } else {
@@ -949,7 +1030,7 @@ int transform_dev(double *z, size_t length, bool inverse, hipStream_t s) {
       fwd_fn f = pick_inv(p.L[i], p.C[i]);
       if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
       ProfScope ps(BCG_K_INV, s);
-      f(false, b.ntiles, s, b);
+      f(false, b.ntiles, 1, s, b);
       HIPCHK(hipGetLastError());
     }
   }
@@ -1130,7 +1211,8 @@ int bcg_shutdown(void) {
   std::lock_guard<std::recursive_mutex> lk(g_lock);
   if (!g.ready) return 0;
   (void)hipStreamSynchronize(g.stream);
-  for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf, &g.desc, &g.margins, &g.counters}) {
+  for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf, &g.desc, &g.margins,
+                    &g.counters, &g.bbufA, &g.bbufB, &g.bblk, &g.bdesc}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->bytes = 0;
@@ -1258,8 +1340,9 @@ int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const s
   // plan every product; one-pass plans are grouped by length class and run as
   // persistent class kernels, everything else goes through the single path
   std::map<int, std::vector<size_t>> classes;
+  std::map<unsigned long long, std::vector<size_t>> mclasses;
   std::vector<Plan> plans(count);
-  std::vector<size_t> single;
+  std::vector<size_t> single, retry;
   for (size_t i = 0; i < count; i++) {
     const uint32_t *pa = A + a_off[i];
     const uint32_t *pb = B + b_off[i];
@@ -1271,8 +1354,57 @@ int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const s
     bool alias = (C + c_off[i] == pa || C + c_off[i] == pb);
     if (plans[i].P == 1 && !alias)
       classes[plans[i].L[0]].push_back(i);
+    else if (!alias)
+      mclasses[plans[i].N].push_back(i);
     else
       single.push_back(i);
+  }
+  // multi-pass classes: all products of one transform length in one launch per pass
+  if (!mclasses.empty()) {
+    size_t nm = 0;
+    for (auto &kv : mclasses) nm += kv.second.size();
+    if (int rc = ensure(g.bdesc, nm * sizeof(MulDesc))) return rc;
+    if (int rc = ensure(g.margins, (nm + count) * sizeof(unsigned int))) return rc;
+    std::vector<MulDesc> all;
+    std::vector<size_t> order;
+    for (auto &kv : mclasses)
+      for (size_t i : kv.second) {
+        MulDesc d;
+        d.a = A + a_off[i];
+        d.b = B + b_off[i];
+        d.c = C + c_off[i];
+        d.al = al[i];
+        d.bl = bl[i];
+        d.cl = cl[i];
+        d.bits = plans[i].bits;
+        d.cbits = coef_bits(plans[i]);
+        all.push_back(d);
+        order.push_back(i);
+      }
+    HIPCHK(hipMemcpyAsync(g.bdesc.p, all.data(), nm * sizeof(MulDesc), hipMemcpyHostToDevice, s));
+    LAUNCH(k_zero_u32, (int)((nm + 255) / 256), 256, s, (unsigned int *)g.margins.p, (int)nm);
+    size_t first = 0;
+    for (auto &kv : mclasses) {
+      std::vector<MulDesc> hd(all.begin() + first, all.begin() + first + kv.second.size());
+      // one plan per class: the products share N, each keeps its own digit size
+      Plan pc = plans[kv.second[0]];
+      if (int rc = run_multipass_class(pc, (const MulDesc *)g.bdesc.p + first, hd,
+                                       (unsigned int *)g.margins.p + first, s))
+        return rc;
+      first += kv.second.size();
+    }
+    std::vector<unsigned int> mb(nm);
+    HIPCHK(hipMemcpyAsync(mb.data(), g.margins.p, nm * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (size_t k = 0; k < nm; k++) {
+      float f;
+      memcpy(&f, &mb[k], sizeof f);
+      if (f >= 0.25f) {
+        retry.push_back(order[k]);
+        continue;
+      }
+      if (f > worst) worst = f;
+    }
   }
   size_t nclass = 0;
   for (auto &kv : classes) nclass += kv.second.size();
@@ -1317,7 +1449,7 @@ int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const s
       float f;
       memcpy(&f, &mb[k], sizeof f);
       if (f >= 0.25f) {
-        single.push_back(order[k]);  // redo with a smaller digit size
+        retry.push_back(order[k]);  // redo with a smaller digit size
         continue;
       }
       if (f > worst) worst = f;
@@ -1325,15 +1457,14 @@ int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const s
   }
   for (size_t i : single) {
     bcg_mul_info inf;
-    const uint32_t *pa = A + a_off[i];
-    const uint32_t *pb = B + b_off[i];
-    bool forced = false;
-    int fb = 0;
-    if (plans[i].N && plans[i].P == 1 && !(C + c_off[i] == pa || C + c_off[i] == pb)) {
-      fb = plans[i].bits - 1;  // came back from a class kernel with margin >= 0.25
-      forced = true;
-    }
-    if (int rc = mul_dev_locked(C + c_off[i], cl[i], pa, al[i], pb, bl[i], forced ? fb : 0, s, &inf)) return rc;
+    if (int rc = mul_dev_locked(C + c_off[i], cl[i], A + a_off[i], al[i], B + b_off[i], bl[i], 0, s, &inf)) return rc;
+    if (inf.margin > worst) worst = inf.margin;
+  }
+  for (size_t i : retry) {
+    bcg_mul_info inf;
+    int fb = plans[i].bits - 1;
+    if (fb < 4) return fail(BCG_E_MARGIN, "rounding margin >= 0.25 at the smallest digit size");
+    if (int rc = mul_dev_locked(C + c_off[i], cl[i], A + a_off[i], al[i], B + b_off[i], bl[i], fb, s, &inf)) return rc;
     if (inf.margin > worst) worst = inf.margin;
   }
   if (worst_margin) *worst_margin = worst;

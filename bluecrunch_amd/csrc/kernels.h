@@ -6,6 +6,16 @@
 
 namespace bcg {
 
+// One product of a batch.
+struct MulDesc {
+  const uint32_t *a;
+  const uint32_t *b;
+  uint32_t *c;
+  unsigned long long al, bl, cl;
+  int bits;
+  int cbits;
+};
+
 struct PassArgs {
   cd *data;                 // complex array of the pass (operand A / product)
   const cd *data_b;         // operand B spectrum array (middle pass)
@@ -23,7 +33,26 @@ struct PassArgs {
   int ntiles;
   unsigned int *margin;     // float bits of the worst |x - round(x)| (atomic max)
   double inv_n;
+  const MulDesc *desc;      // batched launch: product blockIdx.y, else nullptr
+  int operand;              // batched first pass: 0 reads desc.a, 1 reads desc.b
 };
+
+// Batched launches (desc != nullptr): product p = blockIdx.y owns the arrays
+// data + p*N, data_b + p*N, coef + p*2N and margin slot p.
+__device__ __forceinline__ void bind_product(PassArgs &a) {
+  if (!a.desc) return;
+  const unsigned long long p = blockIdx.y;
+  const MulDesc &d = a.desc[p];
+  a.wa = a.operand ? d.b : d.a;
+  a.al = a.operand ? d.bl : d.al;
+  a.wb = d.b;
+  a.bl = d.bl;
+  a.bits = d.bits;
+  if (a.data) a.data += p * a.N;
+  if (a.data_b) a.data_b += p * a.N;
+  if (a.coef) a.coef += p * 2 * a.N;
+  if (a.margin) a.margin += p;
+}
 
 // Tile order that keeps adjacent column tiles on one XCD at the same time:
 // workgroup b runs on XCD b % 8, so tile = (b % 8) * (ntiles / 8) + b / 8.
@@ -87,6 +116,7 @@ __device__ __forceinline__ void margin_reduce(unsigned int *m, double d) {
 template <int L, int C, bool PACK>
 __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   using TC = Tile<L, C>;
+  bind_product(a);
   __shared__ cd lds[TC::LDS_ELEMS];
   const int tid = threadIdx.x;
   const int tile = tile_of_block(a.ntiles);
@@ -138,6 +168,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
 template <int L, int C, bool UNPACK>
 __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
   using TC = Tile<L, C>;
+  bind_product(a);
   __shared__ cd lds[TC::LDS_ELEMS];
   const int tid = threadIdx.x;
   const int tile = tile_of_block(a.ntiles);
@@ -341,6 +372,7 @@ __device__ __forceinline__ void mid_pipeline(const PassArgs &a, cd *lds, unsigne
 template <int L, bool SQUARE, bool PACK, bool UNPACK>
 __global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
   using TC = Tile<L, 1>;
+  bind_product(a);
   __shared__ cd lds[TC::LDS_ELEMS];
   const int tid = threadIdx.x;
   const unsigned long long blk0 = (unsigned long long)blockIdx.x * L;
@@ -375,6 +407,8 @@ constexpr int CWPT = 4;
 constexpr int CT = 256;
 
 struct CarryArgs {
+  const MulDesc *desc;       // batched: product blockIdx.y (local) / blockIdx.x (fix), else nullptr
+  unsigned long long coef_stride;  // batched: coefficients per product (2N)
   const long long *coef;
   unsigned long long ncoef;
   int bits;
@@ -411,9 +445,25 @@ __device__ __forceinline__ void word_partials(const CarryArgs &a, long long wfir
   }
 }
 
+// Batched carries: product p's coefficients at coef + p * coef_stride, its
+// block functions at blk_fn + p * nblk, its words and sizes from desc[p].
+__device__ __forceinline__ void bind_carry(CarryArgs &a, unsigned long long p) {
+  if (!a.desc) return;
+  const MulDesc &d = a.desc[p];
+  a.coef += p * a.coef_stride;
+  a.bits = d.bits;
+  a.cbits = d.cbits;
+  a.out = d.c;
+  a.CL = d.cl;
+  a.blk_fn += p * a.nblk;
+  a.nblk = (d.cl + (unsigned long long)CT * CWPT - 1) / ((unsigned long long)CT * CWPT);
+}
+
 __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
   __shared__ unsigned char fn[CT];
   const int tid = threadIdx.x;
+  bind_carry(a, blockIdx.y);
+  if (blockIdx.x >= a.nblk) return;
   const long long wfirst = ((long long)blockIdx.x * CT + tid) * CWPT;
   unsigned long long P[CWPT + 1];
   if (wfirst < (long long)a.CL) word_partials(a, wfirst, P);
@@ -463,6 +513,7 @@ __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
 __global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
   __shared__ unsigned char fns[1024];
   const int tid = threadIdx.x;
+  bind_carry(a, blockIdx.x);
   const unsigned long long per = (a.nblk + 1023) / 1024;
   const unsigned long long b0 = tid * per;
   unsigned long long b1 = b0 + per;
@@ -597,16 +648,6 @@ __device__ void wg_carry(const long long *coef, long long ncoef, int bits, int c
     prev = P;
   }
 }
-
-// One product of a batch.
-struct MulDesc {
-  const uint32_t *a;
-  const uint32_t *b;
-  uint32_t *c;
-  unsigned long long al, bl, cl;
-  int bits;
-  int cbits;
-};
 
 // All products of one transform length class L (one-pass plans: N = L).
 struct ClassArgs {

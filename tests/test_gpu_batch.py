@@ -42,3 +42,14 @@ def test_batch_benchmark_shape(bc):
     bad = [i for i, ((a, b), c) in enumerate(zip(pairs, out)) if not np.array_equal(c, exact_product(a, b))]
     assert not bad
     assert worst < 0.25
+
+
+def test_batch_multipass_classes(bc):
+    # many two-pass products sharing transform lengths, plus a few odd sizes
+    rng = SplitMix64(5)
+    pairs = [(operand(rng, 9000 + 13 * i), operand(rng, 7000 + 29 * i)) for i in range(40)]
+    pairs += [(operand(rng, 30000), operand(rng, 100)), (operand(rng, 16384), operand(rng, 16384))]
+    out, worst = bc.mul_batch(pairs)
+    for (a, b), c in zip(pairs, out):
+        assert np.array_equal(c, exact_product(a, b))
+    assert worst < 0.25

@@ -25,7 +25,7 @@ EXPORTS = {
     "bluecrunch_gpu.h": [
         "bcg_init", "bcg_shutdown", "bcg_last_error", "bcg_stream", "bcg_sync", "bcg_malloc", "bcg_free",
         "bcg_memcpy_h2d", "bcg_memcpy_d2h", "bcg_mul_dev", "bcg_mul_host", "bcg_mul_dev_bits", "bcg_basecase_dev",
-        "bcg_mul_batch_dev", "bcg_fft_forward_dev", "bcg_fft_inverse_dev", "bcg_fft_pointwise_dev",
+        "bcg_mul_batch_dev", "bcg_muladd_dev", "bcg_muladd_batch_dev", "bcg_fft_forward_dev", "bcg_fft_inverse_dev", "bcg_fft_pointwise_dev",
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
@@ -91,6 +91,8 @@ def lib():
     L.bcg_basecase_dev.argtypes = mul_args + [_vp]
     L.bcg_basecase_host.argtypes = mul_args
     L.bcg_mul_batch_dev.argtypes = [_vp, _vp, _vp] + [ctypes.POINTER(_sz)] * 6 + [_sz, _vp, _f64p]
+    L.bcg_muladd_dev.argtypes = mul_args + [_vp, _sz, _vp, ctypes.POINTER(MulInfo)]
+    L.bcg_muladd_batch_dev.argtypes = [_vp, _vp, _vp, _vp] + [ctypes.POINTER(_sz)] * 8 + [_sz, _vp, _f64p]
     L.bcg_fft_forward_dev.argtypes = [_vp, _sz, _vp]
     L.bcg_fft_inverse_dev.argtypes = [_vp, _sz, _vp]
     L.bcg_fft_pointwise_dev.argtypes = [_vp, _vp, _sz, _vp]
@@ -389,25 +391,44 @@ def sync():
     check(lib().bcg_sync(), "sync")
 
 
-def mul_batch(pairs, cls=None):
-    """Batched products of host (a, b) word-array pairs in one call
-    (bcg_mul_batch_dev); returns (list of product arrays, worst margin)."""
+def mul_batch(pairs, cls=None, adds=None):
+    """Batched products a*b (+ d) of host word arrays in one call
+    (bcg_muladd_batch_dev); returns (list of product arrays, worst margin)."""
     count = len(pairs)
     al = np.array([len(a) for a, _ in pairs], dtype=np.uint64)
     bl = np.array([len(b) for _, b in pairs], dtype=np.uint64)
     cl = (al + bl) if cls is None else np.array(cls, dtype=np.uint64)
-    a_off = np.concatenate([[0], np.cumsum(al)[:-1]]).astype(np.uint64)
-    b_off = np.concatenate([[0], np.cumsum(bl)[:-1]]).astype(np.uint64)
-    c_off = np.concatenate([[0], np.cumsum(cl)[:-1]]).astype(np.uint64)
-    A = np.concatenate([np.ascontiguousarray(a, np.uint32) for a, _ in pairs]) if count else np.zeros(0, np.uint32)
-    B = np.concatenate([np.ascontiguousarray(b, np.uint32) for _, b in pairs]) if count else np.zeros(0, np.uint32)
-    dA, dB = DeviceWords.from_host(A), DeviceWords.from_host(B)
+    offs = lambda n: np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.uint64) if len(n) else n  # noqa: E731
+    a_off, b_off, c_off = offs(al), offs(bl), offs(cl)
+    cat = lambda xs: np.concatenate([np.ascontiguousarray(x, np.uint32) for x in xs]) if xs else np.zeros(0, np.uint32)  # noqa: E731
+    dA, dB = DeviceWords.from_host(cat([a for a, _ in pairs])), DeviceWords.from_host(cat([b for _, b in pairs]))
     dC = DeviceWords(int(cl.sum()))
-    arr = lambda x: np.ascontiguousarray(x, dtype=np.uint64).ctypes.data_as(ctypes.POINTER(_sz))  # noqa: E731
+    if adds is not None:
+        dl = np.array([len(d) for d in adds], dtype=np.uint64)
+        d_off = offs(dl)
+        dD = DeviceWords.from_host(cat(list(adds)))
     keep = [np.ascontiguousarray(x, dtype=np.uint64) for x in (c_off, cl, a_off, al, b_off, bl)]
+    if adds is not None:
+        keep += [np.ascontiguousarray(d_off, np.uint64), np.ascontiguousarray(dl, np.uint64)]
+    ptrs = [k.ctypes.data_as(ctypes.POINTER(_sz)) for k in keep]
+    if adds is None:
+        ptrs += [None, None]
     m = ctypes.c_double()
-    check(lib().bcg_mul_batch_dev(dC.ptr, dA.ptr, dB.ptr, *[k.ctypes.data_as(ctypes.POINTER(_sz)) for k in keep],
-                                  count, None, ctypes.byref(m)), "bcg_mul_batch_dev")
+    check(lib().bcg_muladd_batch_dev(dC.ptr, dA.ptr, dB.ptr, dD.ptr if adds is not None else None, *ptrs,
+                                     count, None, ctypes.byref(m)), "bcg_muladd_batch_dev")
     C = dC.to_host()
     out = [C[int(o):int(o) + int(n)].copy() for o, n in zip(c_off, cl)]
     return out, m.value
+
+
+def muladd(a, b, d, cl=None):
+    """a*b + d (low cl words) on the GPU through bcg_muladd_dev."""
+    a = np.ascontiguousarray(a, np.uint32)
+    b = np.ascontiguousarray(b, np.uint32)
+    d = np.ascontiguousarray(d, np.uint32)
+    cl = a.size + b.size + 1 if cl is None else cl
+    dA, dB, dD, dC = DeviceWords.from_host(a), DeviceWords.from_host(b), DeviceWords.from_host(d), DeviceWords(cl)
+    info = MulInfo()
+    check(lib().bcg_muladd_dev(dC.ptr, cl, dA.ptr, a.size, dB.ptr, b.size, dD.ptr, d.size, None, ctypes.byref(info)),
+          "bcg_muladd_dev")
+    return dC.to_host(), info.as_dict()

@@ -43,7 +43,7 @@ struct Engine {
   hipStream_t stream = nullptr;
   std::map<unsigned long long, RootsDev> roots;  // keyed by N
   std::map<int, cd *> ltab;                      // e^{2 pi i t/L}, keyed by L
-  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, margins, counters, bbufA, bbufB, bblk, bdesc;
+  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, margins, counters, bbufA, bbufB, bblk, bdesc, tdesc;
   bool prof = false;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> free_events;
@@ -238,6 +238,8 @@ int get_ltab(int L, const cd **out) {
 // ------------------------------------------------------------------ plans
 
 bool is_three(unsigned long long n) { return n % 3 == 0; }
+
+int grid_for(unsigned long long n);
 
 // Smallest 2^k or 3*2^k >= n (fft/fft.c:22-41 length classes), at least 64.
 unsigned long long class_ceil(unsigned long long n) {
@@ -553,7 +555,7 @@ int pass_args(const Plan &p, int i, PassArgs &a) {
 
 // Carry resolve: coefficients -> CL words.
 int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbits, uint32_t *out,
-              unsigned long long CL, hipStream_t s) {
+              unsigned long long CL, hipStream_t s, const uint32_t *add = nullptr, unsigned long long addl = 0) {
   if (CL == 0) return 0;
   const unsigned long long wpb = (unsigned long long)CT * CWPT;
   unsigned long long nblk = (CL + wpb - 1) / wpb;
@@ -568,6 +570,8 @@ int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbi
   c.CL = CL;
   c.blk_fn = (unsigned char *)g.blk.p;
   c.nblk = nblk;
+  c.add = add;
+  c.addl = addl;
   {
     ProfScope ps(BCG_K_CARRY, s);
     LAUNCH(k_carry_local, (int)nblk, CT, s, c);
@@ -666,7 +670,7 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
 }
 
 __global__ void k_basecase(const uint32_t *A, unsigned long long AL, const uint32_t *B, unsigned long long BL,
-                           uint32_t *C, unsigned long long CL) {
+                           uint32_t *C, unsigned long long CL, const uint32_t *D, unsigned long long DL) {
   // column sums of 32x32-bit products split into low and high halves
   extern __shared__ unsigned long long colsum[];
   const unsigned long long W = AL + BL;
@@ -688,6 +692,7 @@ __global__ void k_basecase(const uint32_t *A, unsigned long long AL, const uint3
       unsigned long long v = carry;
       if (w < W) v += colsum[2 * w];
       if (w >= 1 && w - 1 < W) v += colsum[2 * (w - 1) + 1];
+      if (D && w < DL) v += D[w];
       C[w] = (uint32_t)v;
       carry = v >> 32;
     }
@@ -695,7 +700,7 @@ __global__ void k_basecase(const uint32_t *A, unsigned long long AL, const uint3
 }
 
 int basecase_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
-                 hipStream_t s) {
+                 hipStream_t s, const uint32_t *D = nullptr, size_t DL = 0) {
   if (AL + BL > 4000) return fail(BCG_E_ARG, "basecase limited to AL + BL <= 4000 words");
   size_t sh = (AL + BL + 1) * 2 * sizeof(unsigned long long);
   // write into a scratch so C may alias A or B
@@ -703,11 +708,18 @@ int basecase_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uin
   {
     ProfScope ps(BCG_K_OTHER, s);
     hipLaunchKernelGGL(k_basecase, dim3(1), dim3(256), sh, s, A, (unsigned long long)AL, B,
-                       (unsigned long long)BL, (uint32_t *)g.zbuf.p, (unsigned long long)CL);
+                       (unsigned long long)BL, (uint32_t *)g.zbuf.p, (unsigned long long)CL, D,
+                       (unsigned long long)DL);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(C, g.zbuf.p, CL * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   return 0;
+}
+
+__global__ void k_copy_words(uint32_t *dst, const uint32_t *src, unsigned long long n, unsigned long long srcn) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) dst[i] = i < srcn ? src[i] : 0u;
 }
 
 int fill_zero(uint32_t *C, size_t CL, hipStream_t s) {
@@ -785,16 +797,18 @@ int run_class(int L, const MulDesc *ddesc, int count, unsigned int *dmargins, in
 
 // One product whose plan is a single pass: the whole product in one workgroup.
 int mul_onepass(const Plan &p, uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
-                hipStream_t s, double *margin) {
+                hipStream_t s, double *margin, const uint32_t *D = nullptr, size_t DL = 0) {
   if (int rc = ensure(g.desc, sizeof(MulDesc))) return rc;
   if (int rc = ensure(g.counters, 64)) return rc;
   MulDesc d;
   d.a = A;
   d.b = B;
   d.c = C;
+  d.d = D;
   d.al = AL;
   d.bl = BL;
   d.cl = CL;
+  d.dl = D ? DL : 0;
   d.bits = p.bits;
   d.cbits = coef_bits(p);
   HIPCHK(hipMemcpyAsync(g.desc.p, &d, sizeof d, hipMemcpyHostToDevice, s));
@@ -882,18 +896,26 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
 }
 
 int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
-                   int force_bits, hipStream_t s, bcg_mul_info *info) {
+                   int force_bits, hipStream_t s, bcg_mul_info *info, const uint32_t *D = nullptr, size_t DL = 0) {
   bcg_mul_info inf;
   memset(&inf, 0, sizeof inf);
   bool square = (A == B && AL == BL);
   inf.square = square ? 1 : 0;
+  if (!D) DL = 0;
+  if (D && C == D) return fail(BCG_E_ARG, "the addend may not alias the output");
   if (AL == 0 || BL == 0 || CL == 0) {
     if (info) *info = inf;
-    return fill_zero(C, CL, s);
+    if (int rc = fill_zero(C, CL, s)) return rc;
+    if (DL && CL) {
+      LAUNCH(k_copy_words, grid_for(CL), 256, s, C, (const uint32_t *)D, (unsigned long long)CL,
+             (unsigned long long)DL);
+      HIPCHK(hipGetLastError());
+    }
+    return 0;
   }
   if (AL + BL < 100 && force_bits <= 0) {
     if (info) *info = inf;
-    return basecase_dev(C, CL, A, AL, B, BL, s);
+    return basecase_dev(C, CL, A, AL, B, BL, s, D, DL);
   }
   Plan p;
   if (int rc = make_plan(AL, BL, force_bits, p)) return rc;
@@ -908,7 +930,7 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
         if (int rc = ensure(g.zbuf, CL * sizeof(uint32_t) + 64)) return rc;
         dst = (uint32_t *)g.zbuf.p;
       }
-      if (int rc = mul_onepass(p, dst, CL, A, AL, B, BL, s, &m)) return rc;
+      if (int rc = mul_onepass(p, dst, CL, A, AL, B, BL, s, &m, D, DL)) return rc;
       inf.bits = p.bits;
       inf.passes = 1;
       inf.points = (size_t)p.N;
@@ -931,7 +953,7 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
       if (int rc = read_margin(s, &m)) return rc;
     }
     if (!alias) {
-      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s)) return rc;
+      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s, D, DL)) return rc;
       if (int rc = read_margin(s, &m)) return rc;
     }
     inf.bits = p.bits;
@@ -950,7 +972,7 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
       continue;
     }
     if (alias) {
-      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s)) return rc;
+      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s, D, DL)) return rc;
     }
     break;
   }
@@ -1212,7 +1234,7 @@ int bcg_shutdown(void) {
   if (!g.ready) return 0;
   (void)hipStreamSynchronize(g.stream);
   for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf, &g.desc, &g.margins,
-                    &g.counters, &g.bbufA, &g.bbufB, &g.bblk, &g.bdesc}) {
+                    &g.counters, &g.bbufA, &g.bbufB, &g.bblk, &g.bdesc, &g.tdesc}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->bytes = 0;
@@ -1331,54 +1353,82 @@ int bcg_basecase_host(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, cons
   return 0;
 }
 
-int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const size_t *c_off, const size_t *cl,
-                      const size_t *a_off, const size_t *al, const size_t *b_off, const size_t *bl, size_t count,
-                      void *stream, double *worst_margin) {
+int bcg_muladd_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const uint32_t *D, const size_t *c_off,
+                         const size_t *cl, const size_t *a_off, const size_t *al, const size_t *b_off, const size_t *bl,
+                         const size_t *d_off, const size_t *dl, size_t count, void *stream, double *worst_margin) {
   ENTER();
   hipStream_t s = pick(stream);
   double worst = 0;
-  // plan every product; one-pass plans are grouped by length class and run as
-  // persistent class kernels, everything else goes through the single path
+  auto desc_of = [&](size_t i, const Plan *pl) {
+    MulDesc d;
+    d.a = A + a_off[i];
+    d.b = B + b_off[i];
+    d.c = C + c_off[i];
+    d.d = (D && dl && dl[i]) ? D + d_off[i] : nullptr;
+    d.al = al[i];
+    d.bl = bl[i];
+    d.cl = cl[i];
+    d.dl = d.d ? dl[i] : 0;
+    d.bits = pl ? pl->bits : 0;
+    d.cbits = pl ? coef_bits(*pl) : 0;
+    return d;
+  };
+  // plan every product: tiny ones (schoolbook, one wave each), one-pass plans
+  // grouped by length class (persistent one-workgroup kernels), multi-pass
+  // plans grouped by transform length (one launch per pass), the rest singly
+  std::vector<size_t> tiny, single, retry;
   std::map<int, std::vector<size_t>> classes;
   std::map<unsigned long long, std::vector<size_t>> mclasses;
   std::vector<Plan> plans(count);
-  std::vector<size_t> single, retry;
   for (size_t i = 0; i < count; i++) {
     const uint32_t *pa = A + a_off[i];
     const uint32_t *pb = B + b_off[i];
-    if (al[i] == 0 || bl[i] == 0 || cl[i] == 0 || al[i] + bl[i] < 100) {
+    uint32_t *pc = C + c_off[i];
+    const uint32_t *pd = (D && dl && dl[i]) ? D + d_off[i] : nullptr;
+    bool alias = (pc == pa || pc == pb);
+    if (pd && pd == pc) return fail(BCG_E_ARG, "an addend may not alias its output");
+    if (al[i] == 0 || bl[i] == 0 || cl[i] == 0) {
       single.push_back(i);
       continue;
     }
+    if (al[i] + bl[i] < 100) {
+      if (alias || cl[i] > 4096)
+        single.push_back(i);
+      else
+        tiny.push_back(i);
+      continue;
+    }
     if (int rc = make_plan(al[i], bl[i], 0, plans[i])) return rc;
-    bool alias = (C + c_off[i] == pa || C + c_off[i] == pb);
-    if (plans[i].P == 1 && !alias)
-      classes[plans[i].L[0]].push_back(i);
-    else if (!alias)
-      mclasses[plans[i].N].push_back(i);
-    else
+    if (alias)
       single.push_back(i);
+    else if (plans[i].P == 1)
+      classes[plans[i].L[0]].push_back(i);
+    else
+      mclasses[plans[i].N].push_back(i);
+  }
+  if (!tiny.empty()) {
+    if (int rc = ensure(g.tdesc, tiny.size() * sizeof(MulDesc))) return rc;
+    std::vector<MulDesc> hd;
+    hd.reserve(tiny.size());
+    for (size_t i : tiny) hd.push_back(desc_of(i, nullptr));
+    HIPCHK(hipMemcpyAsync(g.tdesc.p, hd.data(), hd.size() * sizeof(MulDesc), hipMemcpyHostToDevice, s));
+    {
+      ProfScope ps(BCG_K_OTHER, s);
+      LAUNCH(k_basecase_batch, (int)tiny.size(), 64, s, (const MulDesc *)g.tdesc.p, (int)tiny.size());
+    }
+    HIPCHK(hipGetLastError());
   }
   // multi-pass classes: all products of one transform length in one launch per pass
   if (!mclasses.empty()) {
     size_t nm = 0;
     for (auto &kv : mclasses) nm += kv.second.size();
     if (int rc = ensure(g.bdesc, nm * sizeof(MulDesc))) return rc;
-    if (int rc = ensure(g.margins, (nm + count) * sizeof(unsigned int))) return rc;
+    if (int rc = ensure(g.margins, nm * sizeof(unsigned int))) return rc;
     std::vector<MulDesc> all;
     std::vector<size_t> order;
     for (auto &kv : mclasses)
       for (size_t i : kv.second) {
-        MulDesc d;
-        d.a = A + a_off[i];
-        d.b = B + b_off[i];
-        d.c = C + c_off[i];
-        d.al = al[i];
-        d.bl = bl[i];
-        d.cl = cl[i];
-        d.bits = plans[i].bits;
-        d.cbits = coef_bits(plans[i]);
-        all.push_back(d);
+        all.push_back(desc_of(i, &plans[i]));
         order.push_back(i);
       }
     HIPCHK(hipMemcpyAsync(g.bdesc.p, all.data(), nm * sizeof(MulDesc), hipMemcpyHostToDevice, s));
@@ -1386,8 +1436,7 @@ int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const s
     size_t first = 0;
     for (auto &kv : mclasses) {
       std::vector<MulDesc> hd(all.begin() + first, all.begin() + first + kv.second.size());
-      // one plan per class: the products share N, each keeps its own digit size
-      Plan pc = plans[kv.second[0]];
+      Plan pc = plans[kv.second[0]];  // the products share N and the pass split
       if (int rc = run_multipass_class(pc, (const MulDesc *)g.bdesc.p + first, hd,
                                        (unsigned int *)g.margins.p + first, s))
         return rc;
@@ -1399,11 +1448,10 @@ int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const s
     for (size_t k = 0; k < nm; k++) {
       float f;
       memcpy(&f, &mb[k], sizeof f);
-      if (f >= 0.25f) {
+      if (f >= 0.25f)
         retry.push_back(order[k]);
-        continue;
-      }
-      if (f > worst) worst = f;
+      else if (f > worst)
+        worst = f;
     }
   }
   size_t nclass = 0;
@@ -1417,16 +1465,7 @@ int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const s
     hd.reserve(nclass);
     for (auto &kv : classes)
       for (size_t i : kv.second) {
-        MulDesc d;
-        d.a = A + a_off[i];
-        d.b = B + b_off[i];
-        d.c = C + c_off[i];
-        d.al = al[i];
-        d.bl = bl[i];
-        d.cl = cl[i];
-        d.bits = plans[i].bits;
-        d.cbits = coef_bits(plans[i]);
-        hd.push_back(d);
+        hd.push_back(desc_of(i, &plans[i]));
         order.push_back(i);
       }
     HIPCHK(hipMemcpyAsync(g.desc.p, hd.data(), nclass * sizeof(MulDesc), hipMemcpyHostToDevice, s));
@@ -1448,27 +1487,47 @@ int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const s
     for (size_t k = 0; k < nclass; k++) {
       float f;
       memcpy(&f, &mb[k], sizeof f);
-      if (f >= 0.25f) {
+      if (f >= 0.25f)
         retry.push_back(order[k]);  // redo with a smaller digit size
-        continue;
-      }
-      if (f > worst) worst = f;
+      else if (f > worst)
+        worst = f;
     }
   }
   for (size_t i : single) {
     bcg_mul_info inf;
-    if (int rc = mul_dev_locked(C + c_off[i], cl[i], A + a_off[i], al[i], B + b_off[i], bl[i], 0, s, &inf)) return rc;
+    const uint32_t *pd = (D && dl && dl[i]) ? D + d_off[i] : nullptr;
+    if (int rc = mul_dev_locked(C + c_off[i], cl[i], A + a_off[i], al[i], B + b_off[i], bl[i], 0, s, &inf, pd,
+                                pd ? dl[i] : 0))
+      return rc;
     if (inf.margin > worst) worst = inf.margin;
   }
   for (size_t i : retry) {
     bcg_mul_info inf;
     int fb = plans[i].bits - 1;
     if (fb < 4) return fail(BCG_E_MARGIN, "rounding margin >= 0.25 at the smallest digit size");
-    if (int rc = mul_dev_locked(C + c_off[i], cl[i], A + a_off[i], al[i], B + b_off[i], bl[i], fb, s, &inf)) return rc;
+    const uint32_t *pd = (D && dl && dl[i]) ? D + d_off[i] : nullptr;
+    if (int rc = mul_dev_locked(C + c_off[i], cl[i], A + a_off[i], al[i], B + b_off[i], bl[i], fb, s, &inf, pd,
+                                pd ? dl[i] : 0))
+      return rc;
     if (inf.margin > worst) worst = inf.margin;
   }
+  HIPCHK(hipStreamSynchronize(s));
   if (worst_margin) *worst_margin = worst;
   return 0;
+}
+
+int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const size_t *c_off, const size_t *cl,
+                      const size_t *a_off, const size_t *al, const size_t *b_off, const size_t *bl, size_t count,
+                      void *stream, double *worst_margin) {
+  return bcg_muladd_batch_dev(C, A, B, nullptr, c_off, cl, a_off, al, b_off, bl, nullptr, nullptr, count, stream,
+                              worst_margin);
+}
+
+int bcg_muladd_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                   const uint32_t *D, size_t DL, void *stream, bcg_mul_info *info) {
+  ENTER();
+  if ((AL && !A) || (BL && !B) || (CL && !C) || (DL && !D)) return fail(BCG_E_ARG, "null pointer");
+  return mul_dev_locked(C, CL, A, AL, B, BL, 0, pick(stream), info, D, DL);
 }
 
 int bcg_fft_forward_dev(double *z, size_t length, void *stream) {

@@ -11,7 +11,8 @@ struct MulDesc {
   const uint32_t *a;
   const uint32_t *b;
   uint32_t *c;
-  unsigned long long al, bl, cl;
+  const uint32_t *d;  // optional addend: c = a*b + d (low cl words)
+  unsigned long long al, bl, cl, dl;
   int bits;
   int cbits;
 };
@@ -417,6 +418,8 @@ struct CarryArgs {
   unsigned long long CL;
   unsigned char *blk_fn;     // per block: bit0 = carry out if carry in 0, bit1 = if carry in 1
   unsigned long long nblk;
+  const uint32_t *add;       // optional addend words (value += add), addl words
+  unsigned long long addl;
 };
 
 __device__ __forceinline__ void word_partials(const CarryArgs &a, long long wfirst, unsigned long long (&P)[CWPT + 1]) {
@@ -443,6 +446,13 @@ __device__ __forceinline__ void word_partials(const CarryArgs &a, long long wfir
       P[k] += piece;
     }
   }
+  if (a.add) {
+#pragma unroll
+    for (int k = 0; k <= CWPT; k++) {
+      long long w = wfirst - 1 + k;
+      if (w >= 0 && w < (long long)a.addl) P[k] += a.add[w];
+    }
+  }
 }
 
 // Batched carries: product p's coefficients at coef + p * coef_stride, its
@@ -455,6 +465,8 @@ __device__ __forceinline__ void bind_carry(CarryArgs &a, unsigned long long p) {
   a.cbits = d.cbits;
   a.out = d.c;
   a.CL = d.cl;
+  a.add = d.d;
+  a.addl = d.dl;
   a.blk_fn += p * a.nblk;
   a.nblk = (d.cl + (unsigned long long)CT * CWPT - 1) / ((unsigned long long)CT * CWPT);
 }
@@ -604,7 +616,7 @@ __device__ __forceinline__ unsigned long long word_partial_lds(const long long *
 // LDS, carries resolved with a workgroup scan of carry functions.
 template <int T>
 __device__ void wg_carry(const long long *coef, long long ncoef, int bits, int cbits, uint32_t *out, long long CL,
-                         int tid) {
+                         const uint32_t *add, long long addl, int tid) {
   __shared__ unsigned wave_fn[T / 64 + 1];
   const long long per = (CL + T - 1) / T;
   const long long w0 = (long long)tid * per;
@@ -614,9 +626,11 @@ __device__ void wg_carry(const long long *coef, long long ncoef, int bits, int c
   unsigned f = 2u;  // identity
   {
     unsigned long long prev = word_partial_lds(coef, ncoef, bits, cbits, w0 - 1);
+    if (add && w0 >= 1 && w0 - 1 < addl) prev += add[w0 - 1];
     unsigned c0 = 0, c1 = 1;
     for (long long w = w0; w < w1; w++) {
       unsigned long long P = word_partial_lds(coef, ncoef, bits, cbits, w);
+      if (add && w < addl) P += add[w];
       unsigned long long v = (P & 0xffffffffull) + (prev >> 32);
       c0 = (unsigned)((v + c0) >> 32);
       c1 = (unsigned)((v + c1) >> 32);
@@ -640,8 +654,10 @@ __device__ void wg_carry(const long long *coef, long long ncoef, int bits, int c
   unsigned cin = carry_compose(pre, excl) & 1u;  // carry into word w0 (block carry-in is 0)
   // write the words
   unsigned long long prev = word_partial_lds(coef, ncoef, bits, cbits, w0 - 1);
+  if (add && w0 >= 1 && w0 - 1 < addl) prev += add[w0 - 1];
   for (long long w = w0; w < w1; w++) {
     unsigned long long P = word_partial_lds(coef, ncoef, bits, cbits, w);
+    if (add && w < addl) P += add[w];
     unsigned long long v = (P & 0xffffffffull) + (prev >> 32) + cin;
     out[w] = (uint32_t)v;
     cin = (unsigned)(v >> 32);
@@ -720,8 +736,46 @@ __global__ __launch_bounds__((Tile<L, 1>::T)) void k_mul1(ClassArgs ca) {
     }
     margin_reduce(ca.margins + idx, worst);
     __syncthreads();
-    wg_carry<TC::T>(coef, 2 * L, d.bits, d.cbits, d.c, (long long)d.cl, tid);
+    wg_carry<TC::T>(coef, 2 * L, d.bits, d.cbits, d.c, (long long)d.cl, d.d, (long long)d.dl, tid);
     __syncthreads();
+  }
+}
+
+}  // namespace bcg
+
+namespace bcg {
+
+// Batched schoolbook products (CL < 100 words, AL + BL <= 128): one wave per
+// product; lane w sums column w of 32x32-bit partial products (low and high
+// halves apart), lane 0 then runs the carry chain, adding the optional addend.
+__global__ __launch_bounds__(64) void k_basecase_batch(const MulDesc *desc, int count) {
+  __shared__ unsigned long long lo[128], hi[128];
+  const int p = blockIdx.x;
+  if (p >= count) return;
+  const MulDesc d = desc[p];
+  const int W = (int)(d.al + d.bl);
+  for (int w = threadIdx.x; w < W; w += 64) {
+    unsigned long long l = 0, h = 0;
+    int i0 = w >= (int)d.bl ? w - (int)d.bl + 1 : 0;
+    for (int i = i0; i < (int)d.al && i <= w; i++) {
+      unsigned long long pr = (unsigned long long)d.a[i] * d.b[w - i];
+      l += pr & 0xffffffffull;
+      h += pr >> 32;
+    }
+    lo[w] = l;
+    hi[w] = h;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long carry = 0;
+    for (unsigned long long w = 0; w < d.cl; w++) {
+      unsigned long long v = carry;
+      if (w < (unsigned long long)W) v += lo[w];
+      if (w >= 1 && w - 1 < (unsigned long long)W) v += hi[w - 1];
+      if (d.d && w < d.dl) v += d.d[w];
+      d.c[w] = (uint32_t)v;
+      carry = v >> 32;
+    }
   }
 }
 

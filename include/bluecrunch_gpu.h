@@ -75,6 +75,16 @@ int bcg_mul_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const s
                       const size_t *a_off, const size_t *al, const size_t *b_off, const size_t *bl, size_t count,
                       void *stream, double *worst_margin);
 
+/* Fused multiply-add: C = low CL words of A*B + D. D may not alias C. The
+ * binary-splitting step P = P0*Q1 + P1 (main.c:94-96) in one call. */
+int bcg_muladd_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                   const uint32_t *D, size_t DL, void *stream, bcg_mul_info *info);
+/* Batched multiply-add; D/d_off/dl may be NULL (then the same as
+ * bcg_mul_batch_dev), dl[i] == 0 means no addend for product i. */
+int bcg_muladd_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const uint32_t *D, const size_t *c_off,
+                         const size_t *cl, const size_t *a_off, const size_t *al, const size_t *b_off, const size_t *bl,
+                         const size_t *d_off, const size_t *dl, size_t count, void *stream, double *worst_margin);
+
 /* Reference-compatible transforms on device arrays of interleaved complex
  * doubles (fft.h:15-23 semantics: forward e^{+2 pi i/N}, bit-reversed output,
  * unscaled; inverse takes bit-reversed input, natural output, unscaled). */

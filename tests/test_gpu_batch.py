@@ -53,3 +53,28 @@ def test_batch_multipass_classes(bc):
     for (a, b), c in zip(pairs, out):
         assert np.array_equal(c, exact_product(a, b))
     assert worst < 0.25
+
+
+def test_muladd_single_all_paths(bc):
+    from helpers import words_to_int, int_to_words
+    rng = SplitMix64(11)
+    for al, bl, dl in [(3, 4, 9), (40, 50, 100), (300, 200, 600), (5000, 3000, 8000), (40000, 30000, 70000)]:
+        a, b, d = operand(rng, al), operand(rng, bl), operand(rng, dl)
+        cl = max(al + bl, dl) + 1
+        c, _ = bc.muladd(a, b, d, cl)
+        want = (words_to_int(a) * words_to_int(b) + words_to_int(d)) % (1 << (32 * cl))
+        assert words_to_int(c) == want, (al, bl, dl)
+
+
+def test_muladd_batch_binary_splitting_shape(bc):
+    # P = P0*Q1 + P1 for many nodes of mixed sizes, tiny to two-pass
+    from helpers import words_to_int, int_to_words
+    rng = SplitMix64(12)
+    pairs, adds, cls = [], [], []
+    for n in [1, 1, 2, 3, 7, 20, 49, 60, 128, 700, 2500, 9000, 20000] * 3:
+        a, b, d = operand(rng, n), operand(rng, n + 1), operand(rng, n + 1)
+        pairs.append((a, b)); adds.append(d); cls.append(2 * n + 2)
+    out, worst = bc.mul_batch(pairs, cls=cls, adds=adds)
+    for (a, b), d, c, cl in zip(pairs, adds, out, cls):
+        want = (words_to_int(a) * words_to_int(b) + words_to_int(d)) % (1 << (32 * cl))
+        assert words_to_int(c) == want

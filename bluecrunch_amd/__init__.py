@@ -29,7 +29,8 @@ EXPORTS = {
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
-        "bcg_plan_force", "bcg_autotune", "bcg_prof_enable", "bcg_prof_read", "bcg_prof_reset",
+        "bcg_plan_force", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_prof_enable", "bcg_prof_read",
+        "bcg_prof_reset",
     ],
     "fft.h": [
         "twiddle_table", "twiddle_table_size", "fft_ensure_table", "table_select", "fft_free_table",
@@ -109,6 +110,9 @@ def lib():
     L.bcg_plan_candidates.argtypes = [_sz, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
     L.bcg_plan_force.argtypes = [_sz, i]
     L.bcg_autotune.argtypes = [_sz, _sz, i, i, ctypes.POINTER(i), _f64p, i]
+    L.bcg_e_terms.argtypes = [_sz]
+    L.bcg_e_terms.restype = _sz
+    L.bcg_e_bsr_dev.argtypes = [_sz, ctypes.POINTER(_vp), ctypes.POINTER(_sz), ctypes.POINTER(_sz)]
     L.bcg_prof_enable.argtypes = [i]
     L.bcg_prof_read.argtypes = [i, ctypes.POINTER(ctypes.c_long), _f64p]
     # reference C API
@@ -432,3 +436,20 @@ def muladd(a, b, d, cl=None):
     check(lib().bcg_muladd_dev(dC.ptr, cl, dA.ptr, a.size, dB.ptr, b.size, dD.ptr, d.size, None, ctypes.byref(info)),
           "bcg_muladd_dev")
     return dC.to_host(), info.as_dict()
+
+
+def e_terms(words):
+    """Series terms for `words` words of precision (main.c:38-61)."""
+    return lib().bcg_e_terms(words)
+
+
+def e_bsr(terms):
+    """(P, Q) of sum_{k=1}^{terms} 1/k! = P/Q by binary splitting on the GPU."""
+    p = _vp()
+    pl, ql = _sz(), _sz()
+    check(lib().bcg_e_bsr_dev(terms, ctypes.byref(p), ctypes.byref(pl), ctypes.byref(ql)), "bcg_e_bsr_dev")
+    n = pl.value + ql.value
+    out = np.zeros(max(n, 1), dtype=np.uint32)
+    check(lib().bcg_memcpy_d2h(out.ctypes.data, p.value, 4 * n), "d2h")
+    check(lib().bcg_free(p.value), "free")
+    return out[:pl.value], out[pl.value:n]

@@ -368,25 +368,47 @@ bool factor(unsigned long long N, Plan &p) {
   return true;
 }
 
+// Split cache: factor_candidates is only built once per transform length.
+std::map<unsigned long long, std::vector<Plan>> g_cands;
+
+bool factor_cached(unsigned long long N, Plan &p) {
+  auto it = g_cands.find(N);
+  if (it == g_cands.end()) it = g_cands.emplace(N, factor_candidates(N)).first;
+  const std::vector<Plan> &c = it->second;
+  if (c.empty()) return false;
+  int idx = 0;
+  auto t = g_tuned.find(N);
+  if (t != g_tuned.end() && t->second >= 0 && t->second < (int)c.size()) idx = t->second;
+  int bits = p.bits;
+  unsigned long long dA = p.dA, dB = p.dB;
+  p = c[idx];
+  p.bits = bits;
+  p.dA = dA;
+  p.dB = dB;
+  return true;
+}
+
 int make_plan(unsigned long long AL, unsigned long long BL, int force_bits, Plan &best) {
   best = Plan();
+  unsigned long long bestN = 0;
+  int bestb = 0;
   for (int b = 20; b >= 4; b--) {
     if (force_bits > 0 && b != force_bits) continue;
     unsigned long long dA = digits_of(AL, b), dB = digits_of(BL, b);
     unsigned long long need = dA + dB - 1;
     unsigned long long N = class_ceil((need + 1) / 2);
     if (force_bits <= 0 && b > bmax_for(N)) continue;
-    if (best.N == 0 || N < best.N || (N == best.N && b < best.bits)) {
-      Plan p;
-      p.N = N;
-      p.bits = b;
-      p.dA = dA;
-      p.dB = dB;
-      if (!factor(N, p)) continue;
-      best = p;
+    if (bestN == 0 || N < bestN || (N == bestN && b < bestb)) {
+      bestN = N;
+      bestb = b;
     }
   }
-  if (best.N == 0) return fail(BCG_E_ARG, "no transform plan for this size");
+  if (bestN == 0) return fail(BCG_E_ARG, "no transform plan for this size");
+  best.N = bestN;
+  best.bits = bestb;
+  best.dA = digits_of(AL, bestb);
+  best.dB = digits_of(BL, bestb);
+  if (!factor_cached(bestN, best)) return fail(BCG_E_ARG, "no pass split for this length");
   return 0;
 }
 
@@ -1218,6 +1240,8 @@ __global__ void k_fill_hash(uint32_t *p, unsigned long long n, unsigned long lon
   }
 }
 
+#include "e_pipeline.inc"
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -1784,6 +1808,14 @@ int bcg_autotune(size_t AL, size_t BL, int square, int reps, int *chosen, double
   if (rc) return rc;
   if (chosen) *chosen = best;
   return 0;
+}
+
+size_t bcg_e_terms(size_t words) { return e_terms_of(words); }
+
+int bcg_e_bsr_dev(size_t terms, uint32_t **pq, size_t *pl, size_t *ql) {
+  ENTER();
+  if (!pq || !pl || !ql) return fail(BCG_E_ARG, "null pointer");
+  return bsr_device(terms, pq, pl, ql, g.stream);
 }
 
 int bcg_prof_enable(int on) {

@@ -123,6 +123,15 @@ int bcg_plan_force(size_t N, int index);
  * times_ms (optional) receives up to max_out per-candidate milliseconds. */
 int bcg_autotune(size_t AL, size_t BL, int square, int reps, int *chosen, double *times_ms, int max_out);
 
+/* e = 1 + P/Q (the reference main.c pipeline).
+ * bcg_e_terms: number of series terms for `words` words of precision
+ * (main.c:38-61). bcg_e_bsr_dev: P = sum_{k=1}^{T} (k+1)...T and Q = T! by
+ * level-synchronous binary splitting on the GPU (main.c:63-109, same P and Q);
+ * *pq receives one device allocation holding P (pl words) then Q (ql words),
+ * released with bcg_free. */
+size_t bcg_e_terms(size_t words);
+int bcg_e_bsr_dev(size_t terms, uint32_t **pq, size_t *pl, size_t *ql);
+
 /* Kernel timing with HIP events recorded on the launch stream. While enabled,
  * every kernel the engine launches is bracketed by events; bcg_prof_read
  * returns, per kernel kind, launch count and summed milliseconds. */

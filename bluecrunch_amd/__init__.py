@@ -29,7 +29,7 @@ EXPORTS = {
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
-        "bcg_plan_force", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_prof_enable", "bcg_prof_read",
+        "bcg_plan_force", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_prof_enable", "bcg_prof_read",
         "bcg_prof_reset",
     ],
     "fft.h": [
@@ -113,6 +113,7 @@ def lib():
     L.bcg_e_terms.argtypes = [_sz]
     L.bcg_e_terms.restype = _sz
     L.bcg_e_bsr_dev.argtypes = [_sz, ctypes.POINTER(_vp), ctypes.POINTER(_sz), ctypes.POINTER(_sz)]
+    L.bcg_div_fraction_dev.argtypes = [_vp, _sz, _vp, _sz, _sz, ctypes.POINTER(_vp)]
     L.bcg_prof_enable.argtypes = [i]
     L.bcg_prof_read.argtypes = [i, ctypes.POINTER(ctypes.c_long), _f64p]
     # reference C API
@@ -453,3 +454,15 @@ def e_bsr(terms):
     check(lib().bcg_memcpy_d2h(out.ctypes.data, p.value, 4 * n), "d2h")
     check(lib().bcg_free(p.value), "free")
     return out[:pl.value], out[pl.value:n]
+
+
+def div_fraction(p_words, q_words, nw):
+    """floor(P * 2^(32 nw) / Q) (nw words, a few units of error) on the GPU."""
+    dP, dQ = DeviceWords.from_host(p_words), DeviceWords.from_host(q_words)
+    f = _vp()
+    check(lib().bcg_div_fraction_dev(dP.ptr, len(p_words), dQ.ptr, len(q_words), nw, ctypes.byref(f)),
+          "bcg_div_fraction_dev")
+    out = np.zeros(nw, dtype=np.uint32)
+    check(lib().bcg_memcpy_d2h(out.ctypes.data, f.value, 4 * nw), "d2h")
+    check(lib().bcg_free(f.value), "free")
+    return out

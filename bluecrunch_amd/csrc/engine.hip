@@ -1818,6 +1818,16 @@ int bcg_e_bsr_dev(size_t terms, uint32_t **pq, size_t *pl, size_t *ql) {
   return bsr_device(terms, pq, pl, ql, g.stream);
 }
 
+int bcg_div_fraction_dev(const uint32_t *P, size_t pl, const uint32_t *Q, size_t ql, size_t nw, uint32_t **F) {
+  ENTER();
+  if (!P || !Q || !pl || !ql || !nw || !F) return fail(BCG_E_ARG, "bad arguments");
+  DInt f;
+  if (int rc = d_divide_fraction(P, pl, Q, ql, nw, f, g.stream)) return rc;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  *F = f.p;
+  return 0;
+}
+
 int bcg_prof_enable(int on) {
   ENTER();
   if (!on) prof_collect();

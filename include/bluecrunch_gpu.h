@@ -130,6 +130,11 @@ int bcg_autotune(size_t AL, size_t BL, int square, int reps, int *chosen, double
  * *pq receives one device allocation holding P (pl words) then Q (ql words),
  * released with bcg_free. */
 size_t bcg_e_terms(size_t words);
+/* F = floor(P 2^{32 nw} / Q) (nw words, to within a few units in the last
+ * word) for P < Q, by a Newton reciprocal of Q on the GPU (the role of
+ * bigfloat_rcp/bigfloat_div, bigfloat/rcp.c:15-101, div.c:14-27). *F is a
+ * device allocation released with bcg_free. */
+int bcg_div_fraction_dev(const uint32_t *P, size_t pl, const uint32_t *Q, size_t ql, size_t nw, uint32_t **F);
 int bcg_e_bsr_dev(size_t terms, uint32_t **pq, size_t *pl, size_t *ql);
 
 /* Kernel timing with HIP events recorded on the launch stream. While enabled,

@@ -32,3 +32,12 @@ def test_bsr_exact(bc, T):
     assert words_to_int(Q) == q
     assert words_to_int(P) == p
     assert P[-1] != 0 and Q[-1] != 0
+
+
+@pytest.mark.parametrize("T,nw", [(5, 2), (50, 4), (1000, 40), (30000, 900), (200000, 6000)])
+def test_fraction_division(bc, T, nw):
+    P, Q = bc.e_bsr(T)
+    p, q = words_to_int(P), words_to_int(Q)
+    F = bc.div_fraction(P, Q, nw)
+    want = (p << (32 * nw)) // q
+    assert abs(words_to_int(F) - want) <= 8

@@ -29,7 +29,7 @@ EXPORTS = {
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
-        "bcg_plan_force", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_prof_enable", "bcg_prof_read",
+        "bcg_plan_force", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_prof_enable", "bcg_prof_read",
         "bcg_prof_reset",
     ],
     "fft.h": [
@@ -114,6 +114,8 @@ def lib():
     L.bcg_e_terms.restype = _sz
     L.bcg_e_bsr_dev.argtypes = [_sz, ctypes.POINTER(_vp), ctypes.POINTER(_sz), ctypes.POINTER(_sz)]
     L.bcg_div_fraction_dev.argtypes = [_vp, _sz, _vp, _sz, _sz, ctypes.POINTER(_vp)]
+    L.bcg_radix_dev.argtypes = [_vp, _sz, _sz, _vp]
+    L.bcg_e_digits.argtypes = [_sz, ctypes.c_char_p, _sz, _f64p]
     L.bcg_prof_enable.argtypes = [i]
     L.bcg_prof_read.argtypes = [i, ctypes.POINTER(ctypes.c_long), _f64p]
     # reference C API
@@ -466,3 +468,19 @@ def div_fraction(p_words, q_words, nw):
     check(lib().bcg_memcpy_d2h(out.ctypes.data, f.value, 4 * nw), "d2h")
     check(lib().bcg_free(f.value), "free")
     return out
+
+
+def radix(fraction_words, groups):
+    """Base-1e9 digits of a binary fraction (host words, value F / 2^(32 n))."""
+    dF = DeviceWords.from_host(fraction_words)
+    dD = DeviceWords(groups)
+    check(lib().bcg_radix_dev(dF.ptr, len(fraction_words), groups, dD.ptr), "bcg_radix_dev")
+    return dD.to_host()
+
+
+def e_digits(digits):
+    """("2." + digits decimal digits of e, timings dict) computed on the GPU."""
+    buf = ctypes.create_string_buffer(digits + 3)
+    ms = (ctypes.c_double * 4)()
+    check(lib().bcg_e_digits(digits, buf, digits + 3, ms), "bcg_e_digits")
+    return buf.value.decode(), {"bsr_ms": ms[0], "div_ms": ms[1], "radix_ms": ms[2], "total_ms": ms[3]}

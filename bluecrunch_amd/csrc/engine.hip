@@ -2,6 +2,7 @@
 // and the extern "C" ABI declared in include/bluecrunch_gpu.h.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1826,6 +1827,19 @@ int bcg_div_fraction_dev(const uint32_t *P, size_t pl, const uint32_t *Q, size_t
   HIPCHK(hipStreamSynchronize(g.stream));
   *F = f.p;
   return 0;
+}
+
+int bcg_e_digits(size_t digits, char *out, size_t cap, double *ms) {
+  ENTER();
+  if (!out) return fail(BCG_E_ARG, "null output");
+  return e_digits(digits, out, cap, ms, g.stream);
+}
+
+int bcg_radix_dev(const uint32_t *F, size_t nf, size_t groups, uint32_t *digits) {
+  ENTER();
+  if (!F || !nf || !digits) return fail(BCG_E_ARG, "bad arguments");
+  if (!groups) return 0;
+  return radix_convert(F, nf, groups, digits, g.stream);
 }
 
 int bcg_prof_enable(int on) {

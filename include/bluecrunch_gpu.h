@@ -135,6 +135,15 @@ size_t bcg_e_terms(size_t words);
  * bigfloat_rcp/bigfloat_div, bigfloat/rcp.c:15-101, div.c:14-27). *F is a
  * device allocation released with bcg_free. */
 int bcg_div_fraction_dev(const uint32_t *P, size_t pl, const uint32_t *Q, size_t ql, size_t nw, uint32_t **F);
+/* Base-1e9 digits (most significant first) of the binary fraction F (nf
+ * device words, value F / 2^{32 nf}), `groups` of them into device `digits`
+ * (the role of bigfloat_radix, bigfloat/radix.c:321-360). */
+int bcg_radix_dev(const uint32_t *F, size_t nf, size_t groups, uint32_t *digits);
+/* The whole e pipeline on the GPU: out = "2." + `digits` decimal digits of e,
+ * truncated, NUL-terminated (the e.txt of main.c:111-167); cap >= digits + 3.
+ * ms (optional, 4 entries): binary splitting, division, radix conversion,
+ * total wall milliseconds. */
+int bcg_e_digits(size_t digits, char *out, size_t cap, double *ms);
 int bcg_e_bsr_dev(size_t terms, uint32_t **pq, size_t *pl, size_t *ql);
 
 /* Kernel timing with HIP events recorded on the launch stream. While enabled,

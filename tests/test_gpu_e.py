@@ -41,3 +41,46 @@ def test_fraction_division(bc, T, nw):
     F = bc.div_fraction(P, Q, nw)
     want = (p << (32 * nw)) // q
     assert abs(words_to_int(F) - want) <= 8
+
+
+def e_digits_exact(D):
+    """Truncated decimal digits of e by integer arithmetic: floor(e 10^D)."""
+    scale = 10 ** (D + 10)
+    s, term, k = 0, scale, 0
+    while term:
+        s += term
+        k += 1
+        term //= k
+    return str(s // 10 ** 10)
+
+
+@pytest.mark.parametrize("groups", [1, 5, 31, 32, 33, 100, 1000, 5000])
+def test_radix_exact(bc, groups):
+    # fraction 1/7-ish pattern and a random one, against Python's exact digits
+    rng = np.random.default_rng(groups)
+    n = int(np.ceil(groups * 0.9343253125117121)) + 3
+    F = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    got = bc.radix(F, groups)
+    x = words_to_int(F)
+    want = []
+    for _ in range(groups):
+        x *= 10 ** 9
+        want.append(x >> (32 * n))
+        x &= (1 << (32 * n)) - 1
+    assert list(map(int, got)) == want
+
+
+@pytest.mark.parametrize("D", [1000, 10000, 100000])
+def test_e_digits_md5(bc, golden, D):
+    import hashlib
+    txt, ms = bc.e_digits(D)
+    assert len(txt) == D + 2
+    assert hashlib.md5(txt.encode()).hexdigest() == golden["e_md5"][str(D)]
+    if D <= 10000:
+        assert txt == "2." + e_digits_exact(D)[1:]
+
+
+def test_e_digits_1e6_md5(bc, golden):
+    import hashlib
+    txt, ms = bc.e_digits(1000000)
+    assert hashlib.md5(txt.encode()).hexdigest() == golden["e_md5"]["1000000"]

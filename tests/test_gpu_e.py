@@ -36,10 +36,15 @@ def test_bsr_exact(bc, T):
 
 @pytest.mark.parametrize("T,nw", [(5, 2), (50, 4), (1000, 40), (30000, 900), (200000, 6000)])
 def test_fraction_division(bc, T, nw):
+    # fractional part of P/Q (= e - 2 for large T): divide P - Q by Q
+    from helpers import int_to_words
     P, Q = bc.e_bsr(T)
     p, q = words_to_int(P), words_to_int(Q)
-    F = bc.div_fraction(P, Q, nw)
-    want = (p << (32 * nw)) // q
+    pm = p - q
+    assert 0 <= pm < q
+    n = max(1, (pm.bit_length() + 31) // 32)
+    F = bc.div_fraction(int_to_words(pm, n), Q, nw)
+    want = (pm << (32 * nw)) // q
     assert abs(words_to_int(F) - want) <= 8
 
 

@@ -44,7 +44,8 @@ struct Engine {
   hipStream_t stream = nullptr;
   std::map<unsigned long long, RootsDev> roots;  // keyed by N
   std::map<int, cd *> ltab;                      // e^{2 pi i t/L}, keyed by L
-  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, margins, counters, bbufA, bbufB, bblk, bdesc, tdesc;
+  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, margins, counters, bbufA, bbufB, bblk, bdesc, tdesc,
+      digA, digB;
   bool prof = false;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> free_events;
@@ -578,7 +579,8 @@ int pass_args(const Plan &p, int i, PassArgs &a) {
 
 // Carry resolve: coefficients -> CL words.
 int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbits, uint32_t *out,
-              unsigned long long CL, hipStream_t s, const uint32_t *add = nullptr, unsigned long long addl = 0) {
+              unsigned long long CL, hipStream_t s, const uint32_t *add = nullptr, unsigned long long addl = 0,
+              unsigned cM = 0, unsigned crows = 0) {
   if (CL == 0) return 0;
   const unsigned long long wpb = (unsigned long long)CT * CWPT;
   unsigned long long nblk = (CL + wpb - 1) / wpb;
@@ -595,6 +597,9 @@ int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbi
   c.nblk = nblk;
   c.add = add;
   c.addl = addl;
+  c.coef_T = cM ? 1 : 0;
+  c.cM = cM;
+  c.crows = crows;
   {
     ProfScope ps(BCG_K_CARRY, s);
     LAUNCH(k_carry_local, (int)nblk, CT, s, c);
@@ -618,6 +623,30 @@ int read_margin(hipStream_t s, double *m) {
   *m = (double)f;
   return 0;
 }
+
+int digit_bytes(int bits) { return bits <= 8 ? 1 : (bits <= 16 ? 2 : 4); }
+
+// Operand digits, transposed for the first pass of plan p (see k_digits_T).
+int stage_digits(const Plan &p, const uint32_t *W, unsigned long long WL, DevBuf &dst, hipStream_t s) {
+  const unsigned long long rows = 2ull * (unsigned long long)p.L[0], M = p.M[0];
+  const int ds = digit_bytes(p.bits);
+  if (M % 64 || rows % 64) return fail(BCG_E_ARG, "digit staging needs 64 | M and 64 | 2L");
+  if (int rc = ensure(dst, (size_t)(rows * M) * ds)) return rc;
+  dim3 grid((unsigned)(M / 64), (unsigned)(rows / 64));
+  ProfScope ps(BCG_K_OTHER, s);
+  if (ds == 1)
+    hipLaunchKernelGGL(k_digits_T<uint8_t>, grid, dim3(256), 0, s, W, WL, p.bits, rows, M, (uint8_t *)dst.p);
+  else if (ds == 2)
+    hipLaunchKernelGGL(k_digits_T<uint16_t>, grid, dim3(256), 0, s, W, WL, p.bits, rows, M, (uint16_t *)dst.p);
+  else
+    hipLaunchKernelGGL(k_digits_T<uint32_t>, grid, dim3(256), 0, s, W, WL, p.bits, rows, M, (uint32_t *)dst.p);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// Coefficients of plan p are written transposed when the last pass has
+// narrow column tiles (see unpack_point); the carry reads them back that way.
+bool coef_transposed(const Plan &p) { return p.P >= 2 && p.C[0] < 8; }
 
 // Run the transforms of plan p up to the rounded coefficients (in bufB).
 int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, const uint32_t *B,
@@ -648,15 +677,21 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     HIPCHK(hipGetLastError());
     return 0;
   }
-  // forward outer passes for A (and B)
+  // forward outer passes for A (and B); digits staged transposed first
   for (int op = 0; op < (square ? 1 : 2); op++) {
     cd *buf = op == 0 ? da : db;
+    DevBuf &dig = op == 0 ? g.digA : g.digB;
+    if (int rc = stage_digits(p, op == 0 ? A : B, op == 0 ? AL : BL, dig, s)) return rc;
     for (int i = 0; i < p.P - 1; i++) {
       PassArgs b = a;
       if (int rc = pass_args(p, i, b)) return rc;
       b.data = buf;
       b.wa = op == 0 ? A : B;
       b.al = op == 0 ? AL : BL;
+      if (i == 0) {
+        b.dig = dig.p;
+        b.dig_size = digit_bytes(p.bits);
+      }
       fwd_fn f = pick_fwd(p.L[i], p.C[i]);
       if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
       ProfScope ps(i == 0 ? BCG_K_FWD_PACK : BCG_K_FWD, s);
@@ -683,6 +718,7 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     if (int rc = pass_args(p, i, b)) return rc;
     b.data = da;
     b.coef = coef;
+    b.coef_T = (i == 0 && coef_transposed(p)) ? 1 : 0;
     fwd_fn f = pick_inv(p.L[i], p.C[i]);
     if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
     ProfScope ps(i == 0 ? BCG_K_INV_UNPACK : BCG_K_INV, s);
@@ -976,7 +1012,9 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
       if (int rc = read_margin(s, &m)) return rc;
     }
     if (!alias) {
-      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s, D, DL)) return rc;
+      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s, D, DL,
+                             coef_transposed(p) ? (unsigned)p.M[0] : 0u, 2u * (unsigned)p.L[0]))
+        return rc;
       if (int rc = read_margin(s, &m)) return rc;
     }
     inf.bits = p.bits;
@@ -995,7 +1033,9 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
       continue;
     }
     if (alias) {
-      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s, D, DL)) return rc;
+      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s, D, DL,
+                             coef_transposed(p) ? (unsigned)p.M[0] : 0u, 2u * (unsigned)p.L[0]))
+        return rc;
     }
     break;
   }
@@ -1259,7 +1299,7 @@ int bcg_shutdown(void) {
   if (!g.ready) return 0;
   (void)hipStreamSynchronize(g.stream);
   for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf, &g.desc, &g.margins,
-                    &g.counters, &g.bbufA, &g.bbufB, &g.bblk, &g.bdesc, &g.tdesc}) {
+                    &g.counters, &g.bbufA, &g.bbufB, &g.bblk, &g.bdesc, &g.tdesc, &g.digA, &g.digB}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->bytes = 0;
@@ -1599,8 +1639,7 @@ int bcg_fft_to_int_dev(const double *z, size_t length, uint32_t *A, size_t AL, i
   unsigned long long pts = digits_of(AL, bits);
   if (int rc = ensure(g.zbuf, pts * sizeof(long long))) return rc;
   if (int rc = zero_margin(s)) return rc;
-  unsigned long long avail = pts < length ? pts : (unsigned long long)length;
-  LAUNCH(k_round_points, grid_for(pts), 256, s, (const cd *)z, pts, avail, 1.0 / (double)length, (long long *)g.zbuf.p,
+  LAUNCH(k_round_points, grid_for(pts), 256, s, (const cd *)z, pts, pts, 1.0 / (double)length, (long long *)g.zbuf.p,
          (unsigned int *)g.margin.p);
   HIPCHK(hipGetLastError());
   // r_k are < 2^53; bound the coefficient width by the largest possible value
@@ -1671,11 +1710,12 @@ int bcg_fft_to_int_host(const double *z, size_t length, uint32_t *A, size_t AL, 
     if (margin) *margin = 0;
     return 0;
   }
+  // like fft/convert.c, ceil(32 AL / bits) points are read whatever `length`
+  // is (length only sets the 1/length scale)
   unsigned long long pts = digits_of(AL, bits);
-  unsigned long long avail = pts < length ? pts : (unsigned long long)length;  // points past length read as 0
-  if (int rc = ensure(g.bufA, avail * 16 + 16)) return rc;
+  if (int rc = ensure(g.bufA, pts * 16 + 16)) return rc;
   if (int rc = ensure(g.wC, AL * 4 + 4)) return rc;
-  HIPCHK(hipMemcpyAsync(g.bufA.p, z, avail * 16, hipMemcpyHostToDevice, g.stream));
+  HIPCHK(hipMemcpyAsync(g.bufA.p, z, pts * 16, hipMemcpyHostToDevice, g.stream));
   if (int rc = bcg_fft_to_int_dev((const double *)g.bufA.p, length, (uint32_t *)g.wC.p, AL, bits, margin, g.stream))
     return rc;
   HIPCHK(hipMemcpyAsync(A, g.wC.p, AL * 4, hipMemcpyDeviceToHost, g.stream));

@@ -36,6 +36,9 @@ struct PassArgs {
   double inv_n;
   const MulDesc *desc;      // batched launch: product blockIdx.y, else nullptr
   int operand;              // batched first pass: 0 reads desc.a, 1 reads desc.b
+  const void *dig;          // first pass: digits transposed, dig[m * 2L + t] = digit t*M + m
+  int dig_size;             // bytes per transposed digit (1, 2 or 4)
+  int coef_T;               // last pass: coefficient j at coef[(j % M) * 2L + j / M]
 };
 
 // Batched launches (desc != nullptr): product p = blockIdx.y owns the arrays
@@ -82,6 +85,21 @@ __device__ __forceinline__ cd packed_point(const PassArgs &a, const uint32_t *__
   return cmul(v, omega_pow(a.roots, j));
 }
 
+// Right-angle input point (row t, column m) of the first pass from transposed
+// digits: a_j = dig[m * 2L + t], a_{j+N} = dig[m * 2L + L + t], j = t M + m.
+__device__ __forceinline__ double dig_load(const PassArgs &a, unsigned long long idx) {
+  if (a.dig_size == 1) return (double)((const uint8_t *)a.dig)[idx];
+  if (a.dig_size == 2) return (double)((const uint16_t *)a.dig)[idx];
+  return (double)((const uint32_t *)a.dig)[idx];
+}
+
+__device__ __forceinline__ cd packed_point_T(const PassArgs &a, unsigned long long row, unsigned long long m,
+                                             unsigned long long L) {
+  unsigned long long idx = m * 2 * L + row;
+  cd v = make_double2(dig_load(a, idx), dig_load(a, idx + L));
+  return cmul(v, omega_pow(a.roots, row * a.M + m));
+}
+
 __device__ __forceinline__ void margin_update(unsigned int *m, double d) {
   float f = (float)d;
   // non-negative floats order like their bit patterns
@@ -97,8 +115,15 @@ __device__ __forceinline__ double unpack_point(const PassArgs &a, cd v, unsigned
   double m = fmax(fabs(re - rr), fabs(im - ri));
   // trunc(x + 0.5) like fft/convert.c's (uint64_t)(f + 0.5), clamped at 0
   long long cr = (long long)(re + 0.5), ci = (long long)(im + 0.5);
-  a.coef[j] = cr > 0 ? cr : 0;
-  a.coef[j + a.N] = ci > 0 ? ci : 0;
+  if (a.coef_T) {
+    // j = row M + col: column-major [col][2L] (real part rows, then imaginary)
+    unsigned long long L = a.S / a.M, col = j % a.M, row = j / a.M;
+    a.coef[col * 2 * L + row] = cr > 0 ? cr : 0;
+    a.coef[col * 2 * L + L + row] = ci > 0 ? ci : 0;
+  } else {
+    a.coef[j] = cr > 0 ? cr : 0;
+    a.coef[j + a.N] = ci > 0 ? ci : 0;
+  }
   return m;
 }
 
@@ -140,7 +165,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
       unsigned long long row = (unsigned long long)(q + t * SUB0);
       unsigned long long j = row * a.M + m0 + c;
       if (PACK)
-        x[t] = packed_point(a, a.wa, a.al, j);
+        x[t] = a.dig ? packed_point_T(a, row, m0 + c, L) : packed_point(a, a.wa, a.al, j);
       else
         x[t] = base[j];
     }
@@ -400,6 +425,15 @@ __global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
 
 // ------------------------------------------------------------ carry resolve
 
+// Carry functions of single-bit carry chains: bit0 = carry out for carry in 0,
+// bit1 = carry out for carry in 1. compose(f, g) applies f first, then g.
+__device__ __forceinline__ unsigned carry_compose(unsigned f, unsigned g) {
+  unsigned h0 = (f & 1u) ? (g >> 1) & 1u : g & 1u;
+  unsigned h1 = (f & 2u) ? (g >> 1) & 1u : g & 1u;
+  return h0 | (h1 << 1);
+}
+
+
 // Product words from rounded coefficients: value = sum_j c_j 2^{b j}.
 // Word w collects P_w = sum_j piece(c_j 2^{bj}, word w) (< 2^37); then
 // v_w = lo32(P_w) + hi(P_{w-1}) and a carry chain of single bits resolves v.
@@ -420,7 +454,15 @@ struct CarryArgs {
   unsigned long long nblk;
   const uint32_t *add;       // optional addend words (value += add), addl words
   unsigned long long addl;
+  int coef_T;                // coefficient j at coef[(j % cM) * crows + j / cM]
+  unsigned cM, crows;
 };
+
+__device__ __forceinline__ long long coef_at(const CarryArgs &a, long long j) {
+  if (!a.coef_T) return a.coef[j];
+  unsigned jj = (unsigned)j;
+  return a.coef[(unsigned long long)(jj % a.cM) * a.crows + jj / a.cM];
+}
 
 __device__ __forceinline__ void word_partials(const CarryArgs &a, long long wfirst, unsigned long long (&P)[CWPT + 1]) {
   // words wfirst-1 .. wfirst+CWPT-1 -> P[0..CWPT]
@@ -432,7 +474,7 @@ __device__ __forceinline__ void word_partials(const CarryArgs &a, long long wfir
   long long jhi = hi_bit / a.bits;
   if (jhi >= (long long)a.ncoef) jhi = (long long)a.ncoef - 1;
   for (long long j = jlo; j <= jhi; j++) {
-    unsigned long long c = (unsigned long long)a.coef[j];
+    unsigned long long c = (unsigned long long)coef_at(a, j);
     if (!c) continue;
     long long shift = (long long)a.bits * j - 32 * (wfirst - 1);
 #pragma unroll
@@ -471,52 +513,107 @@ __device__ __forceinline__ void bind_carry(CarryArgs &a, unsigned long long p) {
   a.nblk = (d.cl + (unsigned long long)CT * CWPT - 1) / ((unsigned long long)CT * CWPT);
 }
 
+// Coefficient window of one carry block in LDS: words [W0, W0 + CT*CWPT) need
+// coefficients j with b j in (32 (W0-1) - cbits, 32 (W0 + CT*CWPT)).
+constexpr int CARRY_LDS = 8256;  // int64 slots: enough for b >= 4 and cbits <= 63
+
+__device__ __forceinline__ unsigned long long partial_from_lds(const long long *cl, long long jbase, long long ncoef,
+                                                               int bits, int cbits, long long w) {
+  if (w < 0) return 0ull;
+  long long lo_bit = 32 * w - cbits + 1;
+  long long jlo = lo_bit <= 0 ? 0 : (lo_bit + bits - 1) / bits;
+  long long jhi = (32 * w + 31) / bits;
+  if (jhi >= ncoef) jhi = ncoef - 1;
+  unsigned long long P = 0;
+  for (long long j = jlo; j <= jhi; j++) {
+    unsigned long long c = (unsigned long long)cl[j - jbase];
+    long long sh = (long long)bits * j - 32 * w;
+    if (sh >= 0)
+      P += (unsigned long long)(uint32_t)(c << sh);
+    else if (-sh < 64)
+      P += (unsigned long long)(uint32_t)(c >> (-sh));
+  }
+  return P;
+}
+
+// Words of one block from coefficients: P_w (< 2^37) per word, v_w = lo(P_w) +
+// hi(P_{w-1}), a wave-shuffle scan of per-thread carry functions, words
+// written for block carry-in 0 and the block's function saved for k_carry_fix.
 __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
-  __shared__ unsigned char fn[CT];
+  __shared__ long long cl[CARRY_LDS];
+  __shared__ unsigned wave_fn[CT / 64];
   const int tid = threadIdx.x;
   bind_carry(a, blockIdx.y);
   if (blockIdx.x >= a.nblk) return;
-  const long long wfirst = ((long long)blockIdx.x * CT + tid) * CWPT;
+  const long long W0 = (long long)blockIdx.x * CT * CWPT;
+  const long long wfirst = W0 + (long long)tid * CWPT;
+  // stage the block's coefficient window in LDS (coalesced)
+  long long lo_bit = 32 * (W0 - 1) - a.cbits + 1;
+  long long jbase = lo_bit <= 0 ? 0 : (lo_bit + a.bits - 1) / a.bits;
+  long long jend = (32 * (W0 + (long long)CT * CWPT) - 1) / a.bits;  // inclusive
+  if (jend >= (long long)a.ncoef) jend = (long long)a.ncoef - 1;
+  const long long cnt = jend >= jbase ? jend - jbase + 1 : 0;
+  const bool staged = cnt <= CARRY_LDS;
+  if (staged)
+    for (long long t = tid; t < cnt; t += CT) cl[t] = coef_at(a, jbase + t);
+  __syncthreads();
   unsigned long long P[CWPT + 1];
-  if (wfirst < (long long)a.CL) word_partials(a, wfirst, P);
-  else
-    for (int k = 0; k <= CWPT; k++) P[k] = 0;
+  if (staged) {
+#pragma unroll
+    for (int k = 0; k <= CWPT; k++) {
+      long long w = wfirst - 1 + k;
+      P[k] = (w >= 0 && w < (long long)a.CL) ? partial_from_lds(cl, jbase, (long long)a.ncoef, a.bits, a.cbits, w)
+                                            : 0ull;
+    }
+    if (a.add) {
+#pragma unroll
+      for (int k = 0; k <= CWPT; k++) {
+        long long w = wfirst - 1 + k;
+        if (w >= 0 && w < (long long)a.addl && w < (long long)a.CL) P[k] += a.add[w];
+      }
+    }
+  } else {
+    if (wfirst < (long long)a.CL)
+      word_partials(a, wfirst, P);
+    else
+      for (int k = 0; k <= CWPT; k++) P[k] = 0;
+  }
   if (wfirst == 0) P[0] = 0;
-  // v_w and the thread's carry function f(cin) over its words
   unsigned long long v[CWPT];
 #pragma unroll
   for (int k = 0; k < CWPT; k++) v[k] = (P[k + 1] & 0xffffffffull) + (P[k] >> 32);
-  unsigned f0 = 0, f1 = 1;  // carry after the words, for carry-in 0 / 1
+  unsigned f0 = 0, f1 = 1;
 #pragma unroll
   for (int k = 0; k < CWPT; k++) {
     f0 = (unsigned)((v[k] + f0) >> 32);
     f1 = (unsigned)((v[k] + f1) >> 32);
   }
-  fn[tid] = (unsigned char)(f0 | (f1 << 1));
-  __syncthreads();
-  // exclusive scan of carry functions (block carry-in assumed 0); CT is small,
-  // so thread 0 walks it serially in LDS.
-  __shared__ unsigned char cin_of[CT];
-  if (tid == 0) {
-    unsigned g0 = 0, g1 = 1;
-    for (int t = 0; t < CT; t++) {
-      cin_of[t] = (unsigned char)(g0 | (g1 << 1));
-      unsigned f = fn[t];
-      unsigned n0 = (g0 ? (f >> 1) : f) & 1u;
-      unsigned n1 = (g1 ? (f >> 1) : f) & 1u;
-      g0 = n0;
-      g1 = n1;
-    }
-    a.blk_fn[blockIdx.x] = (unsigned char)(g0 | (g1 << 1));
+  const unsigned f = f0 | (f1 << 1);
+  // inclusive scan of carry functions within the wave, then across waves
+  const int lane = tid & 63, wave = tid >> 6;
+  unsigned inc = f;
+  for (int off = 1; off < 64; off <<= 1) {
+    unsigned gq = __shfl_up(inc, off, 64);
+    if (lane >= off) inc = carry_compose(gq, inc);
   }
+  unsigned excl = __shfl_up(inc, 1, 64);
+  if (lane == 0) excl = 2u;
+  if (lane == 63) wave_fn[wave] = inc;
   __syncthreads();
-  unsigned cin = cin_of[tid] & 1u;
+  unsigned pre = 2u;
+  for (int w = 0; w < wave; w++) pre = carry_compose(pre, wave_fn[w]);
+  if (tid == 0) {
+    unsigned tot = 2u;
+    for (int w = 0; w < CT / 64; w++) tot = carry_compose(tot, wave_fn[w]);
+    a.blk_fn[blockIdx.x] = (unsigned char)tot;
+  }
+  unsigned cin = carry_compose(pre, excl) & 1u;
 #pragma unroll
   for (int k = 0; k < CWPT; k++) {
-    unsigned long long s = v[k] + cin;
+    unsigned long long sum = v[k] + cin;
     long long w = wfirst + k;
-    if (w < (long long)a.CL) a.out[w] = (uint32_t)s;
-    cin = (unsigned)(s >> 32);
+    if (w < (long long)a.CL) a.out[w] = (uint32_t)sum;
+    cin = (unsigned)(sum >> 32);
   }
 }
 
@@ -584,13 +681,6 @@ namespace bcg {
 
 // ------------------------------------------- one-workgroup products (batched)
 
-// Carry functions of single-bit carry chains: bit0 = carry out for carry in 0,
-// bit1 = carry out for carry in 1. compose(f, g) applies f first, then g.
-__device__ __forceinline__ unsigned carry_compose(unsigned f, unsigned g) {
-  unsigned h0 = (f & 1u) ? (g >> 1) & 1u : g & 1u;
-  unsigned h1 = (f & 2u) ? (g >> 1) & 1u : g & 1u;
-  return h0 | (h1 << 1);
-}
 
 // P_w for one word from coefficients held in LDS (see word_partials).
 __device__ __forceinline__ unsigned long long word_partial_lds(const long long *coef, long long ncoef, int bits,
@@ -777,6 +867,28 @@ __global__ __launch_bounds__(64) void k_basecase_batch(const MulDesc *desc, int 
       carry = v >> 32;
     }
   }
+}
+
+}  // namespace bcg
+
+namespace bcg {
+
+// Digits of the operand words transposed for the first pass: the 2L x M
+// digit matrix (digit j = t M + m, row-major in j) written column-major,
+// out[m * rows + t], through 64x64 LDS tiles. Digits t*M + m past the words
+// are zero.
+template <typename DT>
+__global__ __launch_bounds__(256) void k_digits_T(const uint32_t *__restrict__ w, unsigned long long wl, int bits,
+                                                  unsigned long long rows, unsigned long long M, DT *__restrict__ out) {
+  __shared__ DT tile[64][65];
+  const unsigned long long m0 = (unsigned long long)blockIdx.x * 64, t0 = (unsigned long long)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int i = ty; i < 64; i += 4) {
+    unsigned long long j = (t0 + i) * M + m0 + tx;
+    tile[i][tx] = (DT)digit_at(w, wl, j, bits);
+  }
+  __syncthreads();
+  for (int i = ty; i < 64; i += 4) out[(m0 + i) * rows + t0 + tx] = tile[tx][i];
 }
 
 }  // namespace bcg

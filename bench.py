@@ -60,6 +60,12 @@ def read_traffic(kind):
     return None
 
 
+def plan_split(bc, n):
+    """Pass lengths and columns per workgroup the library uses for n points."""
+    passes, L, C = bc.plan_candidates(n)[bc.plan_current(n)]
+    return {"L": list(L), "C": list(C)}
+
+
 def cpu_baseline(words, seed):
     """Oracle _fft_mul on host cores for a bounded sample (words x words)."""
     from oracle import refcpu
@@ -96,6 +102,12 @@ def main():
     ap.add_argument("--cpu-words", type=int, default=1 << 24, help="operand words of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default pass split)")
+    ap.add_argument("--split", default=None,
+                    help="force the pass split, e.g. 64,256,4096 (skips the autotuner); for profiler runs")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB three times with a plain elementwise kernel "
+                         "(reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -121,6 +133,20 @@ def main():
     def step_sq():
         return bc.mul_dev(dS.ptr, 2 * al, dA.ptr, al, dA.ptr, al)
 
+    # untimed set-up: the autotuner times every pass split of this transform
+    # length once and keeps the fastest (the product and the square share N)
+    tune_ms = None
+    if args.split:
+        want = [int(x) for x in args.split.split(",")]
+        n = bc.plan(al, bl)["points"]
+        idx = [i for i, c in enumerate(bc.plan_candidates(n)) if list(c[1]) == want]
+        if not idx:
+            sys.exit(f"no pass split {want} for {n} points")
+        bc.plan_force(n, idx[0])
+    elif not args.no_tune:
+        t_tune = time.perf_counter()
+        bc.autotune(al, bl, reps=2)
+        tune_ms = (time.perf_counter() - t_tune) * 1e3
     for _ in range(args.warmup):
         info = step()
         step_sq()
@@ -206,6 +232,8 @@ def main():
             "points": info["points"],
             "passes": info["passes"],
             "worst_margin": worst,
+            "pass_split": plan_split(bc, info["points"]),
+            "autotune_ms": tune_ms,
         },
         "square": {"ms_per_step": sec_sq * 1e3, "digits_per_s": digits / sec_sq, "worst_margin": s_worst,
                    "bits_per_digit": s_info["bits"], "points": s_info["points"]},
@@ -216,7 +244,24 @@ def main():
         "speedup_vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
         "parity_fingerprint_ok": ok,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
+
+
+def copy_calibration(mib):
+    """Three launches of the library's plain vector copy kernel (bcg_memcpy_d2d)
+    of `mib` MiB: each reads and writes exactly mib MiB, the yardstick that
+    scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import bluecrunch_amd as bc
+
+    words = mib * (1 << 20) // 4
+    src = bc.DeviceWords(words)
+    dst = bc.DeviceWords(words)
+    for _ in range(3):
+        bc.check(bc.lib().bcg_memcpy_d2d(dst.ptr, src.ptr, words * 4), "memcpy_d2d")
+    print(f"copy calibration: 3 x {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

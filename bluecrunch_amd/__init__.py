@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbluecrunch.so")
+# BLUECRUNCH_LIB selects another build of the library (e.g. a variant with
+# different compile-time options); default: the in-tree build.
+LIB_PATH = os.environ.get("BLUECRUNCH_LIB") or os.path.join(HERE, "libbluecrunch.so")
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -24,12 +26,12 @@ _vp = ctypes.c_void_p
 EXPORTS = {
     "bluecrunch_gpu.h": [
         "bcg_init", "bcg_shutdown", "bcg_last_error", "bcg_stream", "bcg_sync", "bcg_malloc", "bcg_free",
-        "bcg_memcpy_h2d", "bcg_memcpy_d2h", "bcg_mul_dev", "bcg_mul_host", "bcg_mul_dev_bits", "bcg_basecase_dev",
+        "bcg_memcpy_h2d", "bcg_memcpy_d2h", "bcg_memcpy_d2d", "bcg_mul_dev", "bcg_mul_host", "bcg_mul_dev_bits", "bcg_basecase_dev",
         "bcg_mul_batch_dev", "bcg_muladd_dev", "bcg_muladd_batch_dev", "bcg_fft_forward_dev", "bcg_fft_inverse_dev", "bcg_fft_pointwise_dev",
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
-        "bcg_plan_force", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_prof_enable", "bcg_prof_read",
+        "bcg_plan_force", "bcg_plan_current", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_prof_enable", "bcg_prof_read",
         "bcg_prof_reset",
     ],
     "fft.h": [
@@ -85,6 +87,7 @@ def lib():
     L.bcg_free.argtypes = [_vp]
     L.bcg_memcpy_h2d.argtypes = [_vp, _vp, _sz]
     L.bcg_memcpy_d2h.argtypes = [_vp, _vp, _sz]
+    L.bcg_memcpy_d2d.argtypes = [_vp, _vp, _sz]
     mul_args = [_vp, _sz, _vp, _sz, _vp, _sz]
     L.bcg_mul_dev.argtypes = mul_args + [_vp, ctypes.POINTER(MulInfo)]
     L.bcg_mul_dev_bits.argtypes = mul_args + [i, _vp, ctypes.POINTER(MulInfo)]
@@ -109,6 +112,7 @@ def lib():
     L.bcg_plan_query.argtypes = [_sz, _sz, ctypes.POINTER(i), ctypes.POINTER(_sz), ctypes.POINTER(i)]
     L.bcg_plan_candidates.argtypes = [_sz, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
     L.bcg_plan_force.argtypes = [_sz, i]
+    L.bcg_plan_current.argtypes = [_sz]
     L.bcg_autotune.argtypes = [_sz, _sz, i, i, ctypes.POINTER(i), _f64p, i]
     L.bcg_e_terms.argtypes = [_sz]
     L.bcg_e_terms.restype = _sz
@@ -368,6 +372,13 @@ def plan_candidates(n):
 
 def plan_force(n, index):
     check(lib().bcg_plan_force(n, index), "plan_force")
+
+
+def plan_current(n):
+    """Index into plan_candidates(n) of the split n-point transforms use now."""
+    r = lib().bcg_plan_current(n)
+    check(min(r, 0), "plan_current")
+    return r
 
 
 def autotune(al, bl, square=False, reps=2):

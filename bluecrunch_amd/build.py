@@ -38,16 +38,29 @@ def _run(cmd):
         raise RuntimeError(f"build step failed ({res.returncode}): {' '.join(cmd)}")
 
 
-def build(verbose=True):
-    os.makedirs(BUILD_DIR, exist_ok=True)
-    eng_o = os.path.join(BUILD_DIR, "engine.o")
-    api_o = os.path.join(BUILD_DIR, "api.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "engine.hip"), "-o", eng_o])
-    _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "api.c"), "-o", api_o])
-    tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", eng_o, api_o, "-o", tmp, "-lm"])
-    os.replace(tmp, LIB)
-    return LIB
+def build(verbose=True, jobs=None, out=None, build_dir=None, defines=()):
+    """Compile every csrc/*.hip unit (in parallel) and api.c, link libbluecrunch.so
+    (or `out`); `defines` are extra -D options for the HIP units."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    bdir = build_dir or BUILD_DIR
+    lib_out = out or LIB
+    os.makedirs(bdir, exist_ok=True)
+    units = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    objs = [os.path.join(bdir, u[:-4] + ".o") for u in units]
+    dflags = [f"-D{d}" for d in defines]
+    cmds = [[HIPCC, *HIP_FLAGS, *dflags, "-c", os.path.join(CSRC, u), "-o", o] for u, o in zip(units, objs)]
+    api_o = os.path.join(bdir, "api.o")
+    cmds.append(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "api.c"), "-o", api_o])
+    jobs = jobs or min(len(cmds), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
+    os.makedirs(os.path.dirname(lib_out), exist_ok=True)
+    tmp = lib_out + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, api_o, "-o", tmp, "-lm"])
+    os.replace(tmp, lib_out)
+    return lib_out
 
 
 if __name__ == "__main__":

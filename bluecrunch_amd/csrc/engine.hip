@@ -13,7 +13,7 @@
 #include <vector>
 
 #include "../../include/bluecrunch_gpu.h"
-#include "kernels.h"
+#include "launch.h"
 
 using namespace bcg;
 
@@ -101,7 +101,7 @@ int init_locked(int device) {
   HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   g.device = device;
   g.ready = true;
-  if (int rc = ensure(g.margin, 256)) return rc;
+  if (int rc = ensure(g.margin, sizeof(unsigned int) * MARGIN_SLOTS * MARGIN_STRIDE)) return rc;
   return 0;
 }
 
@@ -220,6 +220,17 @@ int get_roots(unsigned long long N, Roots *out) {
   out->hi = it->second.hi;
   out->sh = it->second.sh;
   out->four_n = it->second.four_n;
+  // table-free form: 4N = 8 q8, q8 = 2^qa or 3 * 2^qa (N even)
+  out->q8 = (out->four_n % 8 == 0) ? out->four_n / 8 : 0;
+  out->q3 = (out->q8 && out->q8 % 3 == 0) ? 1 : 0;
+  unsigned long long q = out->q3 ? out->q8 / 3 : out->q8;
+  out->qa = 0;
+  while (q > 1) {
+    q >>= 1;
+    out->qa++;
+  }
+  const long double quarter_pi = 0.785398163397448309615660845819875721L;
+  out->step = out->q8 ? (double)(quarter_pi / (long double)out->q8) : 0.0;
   return 0;
 }
 
@@ -310,8 +321,8 @@ bool is_pow2(unsigned long long v) { return v && !(v & (v - 1)); }
 
 // Every pass split of N the kernels support: the innermost (contiguous) pass
 // carries the 3*2^k factor and is <= 8192 (<= 6144); outer passes are powers
-// of two in [64, 8192]; at most three passes. Candidate 0 is the default:
-// the largest inner pass, then the fewest passes.
+// of two in [64, 8192]; at most three passes. Ordered by decreasing inner
+// pass, two-pass before three-pass.
 std::vector<Plan> factor_candidates(unsigned long long N) {
   std::vector<Plan> out;
   const bool three = is_three(N);
@@ -355,10 +366,27 @@ std::vector<Plan> factor_candidates(unsigned long long N) {
 // Tuned choice per transform length: index into factor_candidates(N).
 std::map<unsigned long long, int> g_tuned;
 
+// Untuned choice: a two-pass split whose strided pass still moves >= 8
+// columns (128-byte runs) per row; otherwise the three-pass split with the
+// largest inner pass <= 4096 and the shortest first pass, so every strided
+// pass runs 16 columns wide (narrow strided tiles reach about half the
+// bandwidth of 16-column ones on MI355X).
+int default_candidate(const std::vector<Plan> &c) {
+  for (size_t i = 0; i < c.size(); i++)
+    if (c[i].P <= 2 && (c[i].P == 1 || c[i].C[0] >= 8)) return (int)i;
+  int best = -1;
+  for (size_t i = 0; i < c.size(); i++) {
+    if (c[i].P != 3 || c[i].L[2] > 4096) continue;
+    if (best < 0 || c[i].L[2] > c[best].L[2] || (c[i].L[2] == c[best].L[2] && c[i].L[0] < c[best].L[0]))
+      best = (int)i;
+  }
+  return best < 0 ? 0 : best;
+}
+
 bool factor(unsigned long long N, Plan &p) {
   std::vector<Plan> c = factor_candidates(N);
   if (c.empty()) return false;
-  int idx = 0;
+  int idx = default_candidate(c);
   auto it = g_tuned.find(N);
   if (it != g_tuned.end() && it->second >= 0 && it->second < (int)c.size()) idx = it->second;
   int bits = p.bits;
@@ -378,7 +406,7 @@ bool factor_cached(unsigned long long N, Plan &p) {
   if (it == g_cands.end()) it = g_cands.emplace(N, factor_candidates(N)).first;
   const std::vector<Plan> &c = it->second;
   if (c.empty()) return false;
-  int idx = 0;
+  int idx = default_candidate(c);
   auto t = g_tuned.find(N);
   if (t != g_tuned.end() && t->second >= 0 && t->second < (int)c.size()) idx = t->second;
   int bits = p.bits;
@@ -423,140 +451,6 @@ int make_plan(unsigned long long AL, unsigned long long BL, int force_bits, Plan
 #define LAUNCH2(kern, gx, gy, block, stream, ...) \
   hipLaunchKernelGGL(kern, dim3(gx, gy), dim3(block), 0, stream, __VA_ARGS__)
 
-template <int L, int C>
-int launch_fwd(bool pack, int grid, int gy, hipStream_t s, const PassArgs &a) {
-  if (pack)
-    LAUNCH2((k_fwd<L, C, true>), grid, gy, (Tile<L, C>::T), s, a);
-  else
-    LAUNCH2((k_fwd<L, C, false>), grid, gy, (Tile<L, C>::T), s, a);
-  return 0;
-}
-
-template <int L, int C>
-int launch_inv(bool unpack, int grid, int gy, hipStream_t s, const PassArgs &a) {
-  if (unpack)
-    LAUNCH2((k_inv<L, C, true>), grid, gy, (Tile<L, C>::T), s, a);
-  else
-    LAUNCH2((k_inv<L, C, false>), grid, gy, (Tile<L, C>::T), s, a);
-  return 0;
-}
-
-template <int L>
-int launch_mid(bool sq, bool pack, int grid, int gy, hipStream_t s, const PassArgs &a) {
-  constexpr int T = Tile<L, 1>::T;
-  if (pack) {
-    if (sq)
-      LAUNCH2((k_mid<L, true, true, true>), grid, gy, T, s, a);
-    else
-      LAUNCH2((k_mid<L, false, true, true>), grid, gy, T, s, a);
-  } else {
-    if (sq)
-      LAUNCH2((k_mid<L, true, false, false>), grid, gy, T, s, a);
-    else
-      LAUNCH2((k_mid<L, false, false, false>), grid, gy, T, s, a);
-  }
-  return 0;
-}
-
-// Dispatch over the instantiated (L, C) tiles.
-#define FOR_C(L, X)           \
-  case 1: return X<L, 1>;     \
-  case 2: return X<L, 2>;     \
-  case 4: return X<L, 4>;     \
-  case 8: return X<L, 8>;     \
-  case 16: return X<L, 16>;
-
-typedef int (*fwd_fn)(bool, int, int, hipStream_t, const PassArgs &);
-
-template <int L>
-fwd_fn pick_fwd_c(int C) {
-  switch (C) {
-    case 1: return &launch_fwd<L, 1>;
-    case 2: return (L * 2 <= 8192) ? &launch_fwd<L, (L * 2 <= 8192 ? 2 : 1)> : nullptr;
-    case 4: return (L * 4 <= 8192) ? &launch_fwd<L, (L * 4 <= 8192 ? 4 : 1)> : nullptr;
-    case 8: return (L * 8 <= 8192) ? &launch_fwd<L, (L * 8 <= 8192 ? 8 : 1)> : nullptr;
-    case 16: return (L * 16 <= 8192) ? &launch_fwd<L, (L * 16 <= 8192 ? 16 : 1)> : nullptr;
-  }
-  return nullptr;
-}
-
-template <int L>
-fwd_fn pick_inv_c(int C) {
-  switch (C) {
-    case 1: return &launch_inv<L, 1>;
-    case 2: return (L * 2 <= 8192) ? &launch_inv<L, (L * 2 <= 8192 ? 2 : 1)> : nullptr;
-    case 4: return (L * 4 <= 8192) ? &launch_inv<L, (L * 4 <= 8192 ? 4 : 1)> : nullptr;
-    case 8: return (L * 8 <= 8192) ? &launch_inv<L, (L * 8 <= 8192 ? 8 : 1)> : nullptr;
-    case 16: return (L * 16 <= 8192) ? &launch_inv<L, (L * 16 <= 8192 ? 16 : 1)> : nullptr;
-  }
-  return nullptr;
-}
-
-fwd_fn pick_fwd(int L, int C) {
-  switch (L) {
-    case 64: return pick_fwd_c<64>(C);
-    case 128: return pick_fwd_c<128>(C);
-    case 256: return pick_fwd_c<256>(C);
-    case 512: return pick_fwd_c<512>(C);
-    case 1024: return pick_fwd_c<1024>(C);
-    case 2048: return pick_fwd_c<2048>(C);
-    case 4096: return pick_fwd_c<4096>(C);
-    case 8192: return pick_fwd_c<8192>(C);
-    case 96: return pick_fwd_c<96>(C);
-    case 192: return pick_fwd_c<192>(C);
-    case 384: return pick_fwd_c<384>(C);
-    case 768: return pick_fwd_c<768>(C);
-    case 1536: return pick_fwd_c<1536>(C);
-    case 3072: return pick_fwd_c<3072>(C);
-    case 6144: return pick_fwd_c<6144>(C);
-  }
-  return nullptr;
-}
-
-fwd_fn pick_inv(int L, int C) {
-  switch (L) {
-    case 64: return pick_inv_c<64>(C);
-    case 128: return pick_inv_c<128>(C);
-    case 256: return pick_inv_c<256>(C);
-    case 512: return pick_inv_c<512>(C);
-    case 1024: return pick_inv_c<1024>(C);
-    case 2048: return pick_inv_c<2048>(C);
-    case 4096: return pick_inv_c<4096>(C);
-    case 8192: return pick_inv_c<8192>(C);
-    case 96: return pick_inv_c<96>(C);
-    case 192: return pick_inv_c<192>(C);
-    case 384: return pick_inv_c<384>(C);
-    case 768: return pick_inv_c<768>(C);
-    case 1536: return pick_inv_c<1536>(C);
-    case 3072: return pick_inv_c<3072>(C);
-    case 6144: return pick_inv_c<6144>(C);
-  }
-  return nullptr;
-}
-
-typedef int (*mid_fn)(bool, bool, int, int, hipStream_t, const PassArgs &);
-
-mid_fn pick_mid(int L) {
-  switch (L) {
-    case 64: return &launch_mid<64>;
-    case 128: return &launch_mid<128>;
-    case 256: return &launch_mid<256>;
-    case 512: return &launch_mid<512>;
-    case 1024: return &launch_mid<1024>;
-    case 2048: return &launch_mid<2048>;
-    case 4096: return &launch_mid<4096>;
-    case 8192: return &launch_mid<8192>;
-    case 96: return &launch_mid<96>;
-    case 192: return &launch_mid<192>;
-    case 384: return &launch_mid<384>;
-    case 768: return &launch_mid<768>;
-    case 1536: return &launch_mid<1536>;
-    case 3072: return &launch_mid<3072>;
-    case 6144: return &launch_mid<6144>;
-  }
-  return nullptr;
-}
-
 int coef_bits(const Plan &p) {
   unsigned long long m = p.dA < p.dB ? p.dA : p.dB;
   int cb = 2 * p.bits + ilog2u(m + 1) + 2;
@@ -600,9 +494,10 @@ int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbi
   c.coef_T = cM ? 1 : 0;
   c.cM = cM;
   c.crows = crows;
+  c.lds_slots = carry_slots(bits, cbits);
   {
     ProfScope ps(BCG_K_CARRY, s);
-    LAUNCH(k_carry_local, (int)nblk, CT, s, c);
+    hipLaunchKernelGGL(k_carry_local, dim3((unsigned)nblk), dim3(CT), c.lds_slots * sizeof(long long), s, c);
     LAUNCH(k_carry_fix, 1, 1024, s, c);
   }
   HIPCHK(hipGetLastError());
@@ -610,19 +505,28 @@ int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbi
 }
 
 int zero_margin(hipStream_t s) {
-  LAUNCH(k_zero_u32, 1, 64, s, (unsigned int *)g.margin.p, 1);
+  const int n = MARGIN_SLOTS * MARGIN_STRIDE;
+  LAUNCH(k_zero_u32, (n + 255) / 256, 256, s, (unsigned int *)g.margin.p, n);
   return 0;
 }
 
 int read_margin(hipStream_t s, double *m) {
-  unsigned int bits = 0;
-  HIPCHK(hipMemcpyAsync(&bits, g.margin.p, sizeof bits, hipMemcpyDeviceToHost, s));
+  unsigned int slots[MARGIN_SLOTS * MARGIN_STRIDE];
+  HIPCHK(hipMemcpyAsync(slots, g.margin.p, sizeof slots, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  unsigned int bits = 0;
+  for (int i = 0; i < MARGIN_SLOTS; i++) bits = slots[i * MARGIN_STRIDE] > bits ? slots[i * MARGIN_STRIDE] : bits;
   float f;
   memcpy(&f, &bits, sizeof f);
   *m = (double)f;
   return 0;
 }
+
+// 1: the first pass reads digits from a transposed copy made by k_digits_T;
+// 0: it slices them straight out of the operand words.
+#ifndef BCG_STAGE_DIGITS
+#define BCG_STAGE_DIGITS 0
+#endif
 
 int digit_bytes(int bits) { return bits <= 8 ? 1 : (bits <= 16 ? 2 : 4); }
 
@@ -665,6 +569,7 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
   a.wb = B;
   a.bl = BL;
   a.margin = (unsigned int *)g.margin.p;
+  a.margin_mask = MARGIN_SLOTS - 1;
   if (p.P == 1) {
     if (int rc = pass_args(p, 0, a)) return rc;
     a.data = da;
@@ -673,7 +578,7 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     mid_fn f = pick_mid(p.L[0]);
     if (!f) return fail(BCG_E_ARG, "no kernel for this length");
     ProfScope ps(BCG_K_MID, s);
-    f(square, true, 1, 1, s, a);
+    if (f(square, true, 1, 1, s, a)) return fail(BCG_E_ARG, "kernel rejected this pass");
     HIPCHK(hipGetLastError());
     return 0;
   }
@@ -681,21 +586,22 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
   for (int op = 0; op < (square ? 1 : 2); op++) {
     cd *buf = op == 0 ? da : db;
     DevBuf &dig = op == 0 ? g.digA : g.digB;
-    if (int rc = stage_digits(p, op == 0 ? A : B, op == 0 ? AL : BL, dig, s)) return rc;
+    if (BCG_STAGE_DIGITS)
+      if (int rc = stage_digits(p, op == 0 ? A : B, op == 0 ? AL : BL, dig, s)) return rc;
     for (int i = 0; i < p.P - 1; i++) {
       PassArgs b = a;
       if (int rc = pass_args(p, i, b)) return rc;
       b.data = buf;
       b.wa = op == 0 ? A : B;
       b.al = op == 0 ? AL : BL;
-      if (i == 0) {
+      if (i == 0 && BCG_STAGE_DIGITS) {
         b.dig = dig.p;
         b.dig_size = digit_bytes(p.bits);
       }
       fwd_fn f = pick_fwd(p.L[i], p.C[i]);
       if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
       ProfScope ps(i == 0 ? BCG_K_FWD_PACK : BCG_K_FWD, s);
-      f(i == 0, b.ntiles, 1, s, b);
+      if (f(i == 0, b.ntiles, 1, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
       HIPCHK(hipGetLastError());
     }
   }
@@ -709,7 +615,7 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     mid_fn f = pick_mid(p.L[i]);
     if (!f) return fail(BCG_E_ARG, "no middle kernel for this length");
     ProfScope ps(BCG_K_MID, s);
-    f(square, false, (int)(p.N / (unsigned long long)p.L[i]), 1, s, b);
+    if (f(square, false, (int)(p.N / (unsigned long long)p.L[i]), 1, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
     HIPCHK(hipGetLastError());
   }
   // inverse outer passes, the outermost one rounding into coefficients
@@ -722,7 +628,7 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     fwd_fn f = pick_inv(p.L[i], p.C[i]);
     if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
     ProfScope ps(i == 0 ? BCG_K_INV_UNPACK : BCG_K_INV, s);
-    f(i == 0, b.ntiles, 1, s, b);
+    if (f(i == 0, b.ntiles, 1, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
     HIPCHK(hipGetLastError());
   }
   return 0;
@@ -781,6 +687,14 @@ __global__ void k_copy_words(uint32_t *dst, const uint32_t *src, unsigned long l
   for (; i < n; i += st) dst[i] = i < srcn ? src[i] : 0u;
 }
 
+// Plain streaming copy of n 16-byte vectors (grid-stride).
+__global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src,
+                                                unsigned long long n) {
+  unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) dst[i] = src[i];
+}
+
 int fill_zero(uint32_t *C, size_t CL, hipStream_t s) {
   if (!CL) return 0;
   int grid = (int)((CL + 255) / 256);
@@ -788,37 +702,6 @@ int fill_zero(uint32_t *C, size_t CL, hipStream_t s) {
   LAUNCH(k_fill_u32, grid, 256, s, C, (unsigned long long)CL, 0u);
   HIPCHK(hipGetLastError());
   return 0;
-}
-
-typedef void (*mul1_kernel)(ClassArgs);
-
-template <int L>
-int launch_mul1(int grid, hipStream_t s, const ClassArgs &ca) {
-  LAUNCH((k_mul1<L>), grid, (Tile<L, 1>::T), s, ca);
-  return 0;
-}
-
-typedef int (*mul1_fn)(int, hipStream_t, const ClassArgs &);
-
-mul1_fn pick_mul1(int L) {
-  switch (L) {
-    case 64: return &launch_mul1<64>;
-    case 128: return &launch_mul1<128>;
-    case 256: return &launch_mul1<256>;
-    case 512: return &launch_mul1<512>;
-    case 1024: return &launch_mul1<1024>;
-    case 2048: return &launch_mul1<2048>;
-    case 4096: return &launch_mul1<4096>;
-    case 8192: return &launch_mul1<8192>;
-    case 96: return &launch_mul1<96>;
-    case 192: return &launch_mul1<192>;
-    case 384: return &launch_mul1<384>;
-    case 768: return &launch_mul1<768>;
-    case 1536: return &launch_mul1<1536>;
-    case 3072: return &launch_mul1<3072>;
-    case 6144: return &launch_mul1<6144>;
-  }
-  return nullptr;
 }
 
 // Workgroups resident per CU for a one-pass class kernel (LDS bound).
@@ -902,7 +785,7 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
       fwd_fn f = pick_fwd(p.L[i], p.C[i]);
       if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
       ProfScope ps(i == 0 ? BCG_K_FWD_PACK : BCG_K_FWD, s);
-      f(i == 0, b.ntiles, count, s, b);
+      if (f(i == 0, b.ntiles, count, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
       HIPCHK(hipGetLastError());
     }
   }
@@ -915,7 +798,7 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
     mid_fn f = pick_mid(p.L[i]);
     if (!f) return fail(BCG_E_ARG, "no middle kernel for this length");
     ProfScope ps(BCG_K_MID, s);
-    f(false, false, (int)(p.N / (unsigned long long)p.L[i]), count, s, b);
+    if (f(false, false, (int)(p.N / (unsigned long long)p.L[i]), count, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
     HIPCHK(hipGetLastError());
   }
   for (int i = p.P - 2; i >= 0; i--) {
@@ -926,7 +809,7 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
     fwd_fn f = pick_inv(p.L[i], p.C[i]);
     if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
     ProfScope ps(i == 0 ? BCG_K_INV_UNPACK : BCG_K_INV, s);
-    f(i == 0, b.ntiles, count, s, b);
+    if (f(i == 0, b.ntiles, count, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
     HIPCHK(hipGetLastError());
   }
   // carries, one product per grid row
@@ -946,8 +829,17 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
   c.blk_fn = (unsigned char *)g.bblk.p;
   c.nblk = nblk_max;
   {
+    int bmin = 64, cbmax = 0;
+    for (const MulDesc &d : hdesc) {
+      bmin = d.bits < bmin ? d.bits : bmin;
+      cbmax = d.cbits > cbmax ? d.cbits : cbmax;
+    }
+    c.lds_slots = carry_slots(bmin, cbmax);
+  }
+  {
     ProfScope ps(BCG_K_CARRY, s);
-    LAUNCH2(k_carry_local, (int)nblk_max, count, CT, s, c);
+    hipLaunchKernelGGL(k_carry_local, dim3((unsigned)nblk_max, (unsigned)count), dim3(CT),
+                       c.lds_slots * sizeof(long long), s, c);
     LAUNCH(k_carry_fix, count, 1024, s, c);
   }
   HIPCHK(hipGetLastError());
@@ -1104,7 +996,7 @@ int transform_dev(double *z, size_t length, bool inverse, hipStream_t s) {
       fwd_fn f = pick_fwd(p.L[i], p.C[i]);
       if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
       ProfScope ps(BCG_K_FWD, s);
-      f(false, b.ntiles, 1, s, b);
+      if (f(false, b.ntiles, 1, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
       HIPCHK(hipGetLastError());
     }
   } else {
@@ -1115,7 +1007,7 @@ int transform_dev(double *z, size_t length, bool inverse, hipStream_t s) {
       fwd_fn f = pick_inv(p.L[i], p.C[i]);
       if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
       ProfScope ps(BCG_K_INV, s);
-      f(false, b.ntiles, 1, s, b);
+      if (f(false, b.ntiles, 1, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
       HIPCHK(hipGetLastError());
     }
   }
@@ -1217,7 +1109,7 @@ __global__ void k_round_points(const cd *z, unsigned long long pts, unsigned lon
     long long r = (long long)(f + 0.5);
     coef[i] = r > 0 ? r : 0;
   }
-  margin_reduce(margin, worst);
+  margin_reduce(margin, worst, MARGIN_SLOTS - 1);
 }
 
 int grid_for(unsigned long long n) {
@@ -1357,6 +1249,25 @@ int bcg_memcpy_h2d(void *dst, const void *src, size_t bytes) {
 int bcg_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   ENTER();
   HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int bcg_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  ENTER();
+  if (bytes % 4) return fail(BCG_E_ARG, "d2d copy size must be a multiple of 4 bytes");
+  if (!bytes) return 0;
+  if (!dst || !src) return fail(BCG_E_ARG, "null pointer");
+  if (bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0) {
+    const unsigned long long n = bytes / 16;
+    unsigned long long grid = (n + 255) / 256;
+    if (grid > 16384) grid = 16384;
+    LAUNCH(k_copy16, (int)grid, 256, g.stream, (uint4 *)dst, (const uint4 *)src, n);
+  } else {
+    const unsigned long long n = bytes / 4;
+    LAUNCH(k_copy_words, grid_for(n), 256, g.stream, (uint32_t *)dst, (const uint32_t *)src, n, n);
+  }
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g.stream));
   return 0;
 }
@@ -1794,6 +1705,18 @@ int bcg_plan_force(size_t N, int index) {
   if (index >= (int)factor_candidates(N).size()) return fail(BCG_E_ARG, "candidate index out of range");
   g_tuned[N] = index;
   return 0;
+}
+
+int bcg_plan_current(size_t N) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  Plan p;
+  if (!factor_cached(N, p)) return fail(BCG_E_ARG, "no pass split for this length");
+  auto it = g_cands.find(N);
+  for (size_t i = 0; i < it->second.size(); i++) {
+    const Plan &c = it->second[i];
+    if (c.P == p.P && c.L[0] == p.L[0] && c.L[1] == p.L[1] && c.L[2] == p.L[2]) return (int)i;
+  }
+  return fail(BCG_E_ARG, "no pass split for this length");
 }
 
 int bcg_autotune(size_t AL, size_t BL, int square, int reps, int *chosen, double *times_ms, int max_out) {

@@ -94,16 +94,103 @@ __device__ __forceinline__ unsigned freq_of_row(unsigned r) {
 // Root-of-unity tables for one transform length N: Omega = e^{2 pi i / 4N}.
 //   Omega^u = hi[u >> sh] * lo[u & (2^sh - 1)],  u in [0, 4N)
 //   tl_L[t] = e^{2 pi i t / L} for t < L/2, one table per pass length L.
+//   Without tables (omega_direct): 4N = 8 q8 with q8 = 2^qa or 3 * 2^qa and
+//   step = (pi/4) / q8 rounded to double.
 struct Roots {
   const cd *lo;
   const cd *hi;
   int sh;
   unsigned long long four_n;
+  unsigned long long q8;
+  int qa;
+  int q3;
+  double step;
 };
 
 __device__ __forceinline__ cd omega_pow(const Roots &r, unsigned long long u) {
   return cmul(r.hi[u >> r.sh], r.lo[u & ((1ull << r.sh) - 1)]);
 }
+
+// Omega^u for 0 <= u < 4N, computed in registers. The octant o = floor(u / q8)
+// is split off exactly in integers; the rest is an angle y in [0, pi/4]
+// (measured back from the next octant boundary when o is odd) whose cosine and
+// sine come from their Taylor series through y^16 and y^17 (truncation below
+// 3e-18 at y = pi/4); octant symmetries then place the point. No memory access,
+// so twiddles of strided passes cost ALU work instead of scattered loads.
+__host__ __device__ __forceinline__ cd omega_direct(const Roots &r, unsigned long long u) {
+  const unsigned x = (unsigned)(u >> r.qa);             // < 8, or < 24 when q8 = 3 * 2^qa
+  const unsigned o = r.q3 ? (x * 11u) >> 5 : x;         // floor(x / 3) for x < 24
+  unsigned long long rem = u - (unsigned long long)o * r.q8;
+  if (o & 1) rem = r.q8 - rem;
+  const double y = (double)rem * r.step;
+  const double t = y * y;
+  double ps = 1.0 / 355687428096000.0;                  // 1/17!
+  ps = fma(ps, t, -1.0 / 1307674368000.0);
+  ps = fma(ps, t, 1.0 / 6227020800.0);
+  ps = fma(ps, t, -1.0 / 39916800.0);
+  ps = fma(ps, t, 1.0 / 362880.0);
+  ps = fma(ps, t, -1.0 / 5040.0);
+  ps = fma(ps, t, 1.0 / 120.0);
+  ps = fma(ps, t, -1.0 / 6.0);
+  const double sn = fma(y * t, ps, y);
+  double pc = 1.0 / 20922789888000.0;                   // 1/16!
+  pc = fma(pc, t, -1.0 / 87178291200.0);
+  pc = fma(pc, t, 1.0 / 479001600.0);
+  pc = fma(pc, t, -1.0 / 3628800.0);
+  pc = fma(pc, t, 1.0 / 40320.0);
+  pc = fma(pc, t, -1.0 / 720.0);
+  pc = fma(pc, t, 1.0 / 24.0);
+  pc = fma(pc, t, -0.5);
+  const double cs = fma(t, pc, 1.0);
+  // o = 0..7: (c,s) (s,c) (-s,c) (-c,s) (-c,-s) (-s,-c) (s,-c) (c,-s)
+  const bool swap = ((o + 1) & 2) != 0;
+  double re = swap ? sn : cs, im = swap ? cs : sn;
+  if ((o + 2) & 4) re = -re;
+  if (o & 4) im = -im;
+  return make_double2(re, im);
+}
+
+// e^{2 pi i h / 64}, h = 0..15 (compile-time h at every call site).
+__device__ __forceinline__ cd w64(int h) {
+  const double c[16] = {1.0,
+                        0.995184726672196886245,
+                        0.980785280403230449126,
+                        0.956940335732208864936,
+                        0.923879532511286756128,
+                        0.881921264348355029713,
+                        0.831469612302545237079,
+                        0.773010453362736960811,
+                        0.707106781186547524401,
+                        0.634393284163645498215,
+                        0.555570233019602224743,
+                        0.471396736825997648556,
+                        0.382683432365089771728,
+                        0.290284677254462367636,
+                        0.195090322016128267848,
+                        0.098017140329560601994};
+  return make_double2(c[h & 15], c[(16 - h) & 15] * (h ? 1.0 : 0.0));
+}
+
+// Sixteen twiddles base * g^j with j = brev4(i) for i = 0, 1, ..., 15 (the
+// frequency order of a thread's rows): base, g, g^2, g^4 and g^8 come from
+// omega_direct, the other eleven are products along the bits of j, at most
+// four deep. at(i) must be called with i = 0..15 in increasing order.
+struct Walk16 {
+  cd base, g1, g2, g4, g8, p3, p2, p1;
+  __device__ __forceinline__ void init(const Roots &r, unsigned long long u0, unsigned long long du) {
+    base = omega_direct(r, u0);
+    g1 = omega_direct(r, du);
+    g2 = omega_direct(r, 2 * du);
+    g4 = omega_direct(r, 4 * du);
+    g8 = omega_direct(r, 8 * du);
+  }
+  __device__ __forceinline__ cd at(int i) {
+    if ((i & 7) == 0) p3 = (i & 8) ? cmul(base, g1) : base;
+    if ((i & 3) == 0) p2 = (i & 4) ? cmul(p3, g2) : p3;
+    if ((i & 1) == 0) p1 = (i & 2) ? cmul(p2, g4) : p2;
+    return (i & 1) ? cmul(p1, g8) : p1;
+  }
+};
 
 // ---------------------------------------------------------------- butterflies
 

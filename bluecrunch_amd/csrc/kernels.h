@@ -33,6 +33,7 @@ struct PassArgs {
   int bits;                 // digit size
   int ntiles;
   unsigned int *margin;     // float bits of the worst |x - round(x)| (atomic max)
+  int margin_mask;          // single products: slot (block & mask) * MARGIN_STRIDE; batched: 0
   double inv_n;
   const MulDesc *desc;      // batched launch: product blockIdx.y, else nullptr
   int operand;              // batched first pass: 0 reads desc.a, 1 reads desc.b
@@ -78,11 +79,16 @@ __device__ __forceinline__ double digit_at(const uint32_t *__restrict__ w, unsig
   return (double)((v >> sh) & ((1ull << bits) - 1ull));
 }
 
+// Digit pair (a_j, a_{j+N}) of right-angle input point j.
+__device__ __forceinline__ cd packed_digits(const PassArgs &a, const uint32_t *__restrict__ w,
+                                            unsigned long long wl, unsigned long long j) {
+  return make_double2(digit_at(w, wl, j, a.bits), digit_at(w, wl, j + a.N, a.bits));
+}
+
 // Right-angle input point j: (a_j + i a_{j+N}) * Omega^j.
 __device__ __forceinline__ cd packed_point(const PassArgs &a, const uint32_t *__restrict__ w, unsigned long long wl,
                                            unsigned long long j) {
-  cd v = make_double2(digit_at(w, wl, j, a.bits), digit_at(w, wl, j + a.N, a.bits));
-  return cmul(v, omega_pow(a.roots, j));
+  return cmul(packed_digits(a, w, wl, j), omega_pow(a.roots, j));
 }
 
 // Right-angle input point (row t, column m) of the first pass from transposed
@@ -93,11 +99,10 @@ __device__ __forceinline__ double dig_load(const PassArgs &a, unsigned long long
   return (double)((const uint32_t *)a.dig)[idx];
 }
 
-__device__ __forceinline__ cd packed_point_T(const PassArgs &a, unsigned long long row, unsigned long long m,
-                                             unsigned long long L) {
+__device__ __forceinline__ cd packed_digits_T(const PassArgs &a, unsigned long long row, unsigned long long m,
+                                              unsigned long long L) {
   unsigned long long idx = m * 2 * L + row;
-  cd v = make_double2(dig_load(a, idx), dig_load(a, idx + L));
-  return cmul(v, omega_pow(a.roots, row * a.M + m));
+  return make_double2(dig_load(a, idx), dig_load(a, idx + L));
 }
 
 __device__ __forceinline__ void margin_update(unsigned int *m, double d) {
@@ -106,32 +111,46 @@ __device__ __forceinline__ void margin_update(unsigned int *m, double d) {
   atomicMax(m, __float_as_uint(f));
 }
 
-// Round one output point j (value w_j of the unscaled inverse) into
-// coefficients c_j (real part) and c_{j+N} (imaginary part); returns margin.
-__device__ __forceinline__ double unpack_point(const PassArgs &a, cd v, unsigned long long j) {
-  cd f = cmulc(v, omega_pow(a.roots, j));
+// Round one output point j = row * M + col, already multiplied by
+// Omega^{-j} (f = w_j Omega^{-j}, unscaled), into coefficients c_j (real part)
+// and c_{j+N} (imaginary part); returns the rounding margin. L is the pass
+// length of the transposed layout (coef_T only).
+__device__ __forceinline__ double round_store(const PassArgs &a, cd f, unsigned long long row,
+                                              unsigned long long col, unsigned long long L) {
   double re = f.x * a.inv_n, im = f.y * a.inv_n;
   double rr = rint(re), ri = rint(im);
   double m = fmax(fabs(re - rr), fabs(im - ri));
   // trunc(x + 0.5) like fft/convert.c's (uint64_t)(f + 0.5), clamped at 0
   long long cr = (long long)(re + 0.5), ci = (long long)(im + 0.5);
   if (a.coef_T) {
-    // j = row M + col: column-major [col][2L] (real part rows, then imaginary)
-    unsigned long long L = a.S / a.M, col = j % a.M, row = j / a.M;
+    // column-major [col][2L] (real part rows, then imaginary)
     a.coef[col * 2 * L + row] = cr > 0 ? cr : 0;
     a.coef[col * 2 * L + L + row] = ci > 0 ? ci : 0;
   } else {
+    const unsigned long long j = row * a.M + col;
     a.coef[j] = cr > 0 ? cr : 0;
     a.coef[j + a.N] = ci > 0 ? ci : 0;
   }
   return m;
 }
 
-// Wave-level max then one atomic per wave.
-__device__ __forceinline__ void margin_reduce(unsigned int *m, double d) {
+// Whole-point form for single-pass plans (M = 1): j = row.
+__device__ __forceinline__ double unpack_point(const PassArgs &a, cd v, unsigned long long j) {
+  return round_store(a, cmulc(v, omega_pow(a.roots, j)), j, 0, 0);
+}
+
+// Margin slots: a single product spreads its atomics over MARGIN_SLOTS
+// addresses MARGIN_STRIDE words apart (one 128-byte line each), so tens of
+// thousands of workgroups do not serialise on one address; the host takes the
+// maximum over the slots.
+constexpr int MARGIN_SLOTS = 64;
+constexpr int MARGIN_STRIDE = 32;
+
+// Wave-level max then one atomic per wave, into slot (blockIdx.x & mask).
+__device__ __forceinline__ void margin_reduce(unsigned int *m, double d, int mask = 0) {
   float f = (float)d;
   for (int off = 32; off > 0; off >>= 1) f = fmaxf(f, __shfl_xor(f, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(m, __float_as_uint(f));
+  if ((threadIdx.x & 63) == 0) atomicMax(m + (blockIdx.x & mask) * MARGIN_STRIDE, __float_as_uint(f));
 }
 
 // ------------------------------------------------------------ forward pass
@@ -154,20 +173,66 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   constexpr int R0 = 1 << RB0;
   constexpr int PER0 = TC::EPT / R0;
   constexpr int SUB0 = L / R0;
+  static_assert(!PACK || !TC::THREE, "packing pass is a power of two");
+  // Thread tid owns column c = tid % C and rows r0 + h * L/16, r0 = tid / C,
+  // h = 0..15 (T = L*C/16): row r0 + h*L/16 has frequency 16 brev(r0) + brev4(h).
+  const int c = tid % C;
+  const int r0 = tid / C;
+  const unsigned long long m = m0 + c;
+  cd wpack = make_double2(1.0, 0.0);
+  // right-angle weight Omega^{row M + m} = Omega^{r0 M + m} e^{2 pi i h / 64}
+  if (PACK) wpack = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
+  // PACK without a transposed digit copy: the words under each of the 2L row
+  // segments (row t, half h: digits t M + m0 + h N + [0, C)) are staged in LDS
+  // with coalesced loads, then every thread slices its 32 digits out of LDS.
+  uint32_t dre[PACK ? 16 : 1], dim_[PACK ? 16 : 1];
+  if constexpr (PACK) {
+    if (!a.dig) {
+      uint32_t *lw = reinterpret_cast<uint32_t *>(lds);
+      const int b = a.bits;
+      const unsigned SW = (unsigned)(C * b + 62) >> 5;  // words spanned by C digits
+      const unsigned inv = 0xffffffffu / SW + 1u;        // ceil(2^32 / SW)
+      const unsigned n = 2u * L * SW;
+      for (unsigned e = tid; e < n; e += TC::T) {
+        const unsigned seg = __umulhi(e, inv), k = e - seg * SW;
+        const unsigned long long j0 = (unsigned long long)(seg & (L - 1)) * a.M + m0 + (seg >= (unsigned)L ? a.N : 0ull);
+        const unsigned long long wi = ((j0 * (unsigned long long)b) >> 5) + k;
+        lw[e] = wi < a.al ? a.wa[wi] : 0u;
+      }
+      __syncthreads();
+      const unsigned mask = (1u << b) - 1u;
+#pragma unroll
+      for (int h = 0; h < 16; h++) {
+        const unsigned row = (unsigned)r0 + (unsigned)h * (L / 16);
+#pragma unroll
+        for (int half = 0; half < 2; half++) {
+          const unsigned seg = row + (half ? (unsigned)L : 0u);
+          // bit offset of digit (row, m) inside its segment's first word (mod 2^32 arithmetic)
+          const unsigned j0lo = (unsigned)(row * (unsigned)a.M + (unsigned)m0 + (half ? (unsigned)a.N : 0u));
+          const unsigned off = ((j0lo * (unsigned)b) & 31u) + (unsigned)(c * b);
+          const unsigned k = seg * SW + (off >> 5);
+          const unsigned long long v = (unsigned long long)lw[k] | ((unsigned long long)lw[k + 1] << 32);
+          const uint32_t d = (uint32_t)(v >> (off & 31u)) & mask;
+          if (half) dim_[h] = d; else dre[h] = d;
+        }
+      }
+      __syncthreads();
+    }
+  }
 #pragma unroll
   for (int i = 0; i < PER0; i++) {
-    int bf = tid + i * TC::T;
-    int c = bf % C;
-    int q = bf / C;
+    int q = r0 + i * (TC::T / C);
     cd x[R0];
 #pragma unroll
     for (int t = 0; t < R0; t++) {
       unsigned long long row = (unsigned long long)(q + t * SUB0);
-      unsigned long long j = row * a.M + m0 + c;
-      if (PACK)
-        x[t] = a.dig ? packed_point_T(a, row, m0 + c, L) : packed_point(a, a.wa, a.al, j);
-      else
-        x[t] = base[j];
+      if constexpr (PACK) {
+        const int h = i + t * PER0;
+        cd v = a.dig ? packed_digits_T(a, row, m, L) : make_double2((double)dre[h], (double)dim_[h]);
+        x[t] = cmul(v, cmul(wpack, w64(h)));
+      } else {
+        x[t] = base[row * a.M + m];
+      }
     }
     dif_levels<RB0>(x, a.tl, q, 1);
 #pragma unroll
@@ -175,15 +240,28 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   }
   __syncthreads();
   forward_rest<L, C>(lds, a.tl, tid);
-  for (int e = tid; e < TC::E; e += TC::T) {
-    int row = e / C, c = e % C;
-    unsigned long long m = m0 + c;
-    cd v = lds[pad16(e)];
-    if (a.M > 1) {
-      unsigned long long k = freq_of_row<L>((unsigned)row);
-      v = cmul(v, omega_pow(a.roots, m * k * a.tw_mul));
+  if constexpr (TC::THREE) {
+    for (int e = tid; e < TC::E; e += TC::T) {
+      int row = e / C, cc = e % C;
+      unsigned long long mm = m0 + cc;
+      cd v = lds[pad16(e)];
+      if (a.M > 1) {
+        unsigned long long k = freq_of_row<L>((unsigned)row);
+        v = cmul(v, omega_pow(a.roots, mm * k * a.tw_mul));
+      }
+      base[(unsigned long long)row * a.M + mm] = v;
     }
-    base[(unsigned long long)row * a.M + m] = v;
+  } else {
+    // pass twiddle e^{2 pi i m k / S} = Omega^{tw_mul m k}
+    Walk16 w;
+    if (a.M > 1) w.init(a.roots, a.tw_mul * m * 16ull * brev((unsigned)r0, TC::LOG - 4), a.tw_mul * m);
+#pragma unroll
+    for (int h = 0; h < 16; h++) {
+      const int row = r0 + h * (L / 16);
+      cd v = lds[pad16(row * C + c)];
+      if (a.M > 1) v = cmul(v, w.at(h));
+      base[(unsigned long long)row * a.M + m] = v;
+    }
   }
 }
 
@@ -202,15 +280,32 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
   const unsigned long long ob = (unsigned long long)tile / tpb;
   const unsigned long long m0 = ((unsigned long long)tile % tpb) * C;
   cd *base = a.data + ob * a.S;
-  for (int e = tid; e < TC::E; e += TC::T) {
-    int row = e / C, c = e % C;
-    unsigned long long m = m0 + c;
-    cd v = base[(unsigned long long)row * a.M + m];
-    if (a.M > 1) {
-      unsigned long long k = freq_of_row<L>((unsigned)row);
-      v = cmulc(v, omega_pow(a.roots, m * k * a.tw_mul));
+  static_assert(!UNPACK || !TC::THREE, "unpacking pass is a power of two");
+  const int c = tid % C;
+  const int r0 = tid / C;
+  const unsigned long long m = m0 + c;
+  if constexpr (TC::THREE) {
+    for (int e = tid; e < TC::E; e += TC::T) {
+      int row = e / C, cc = e % C;
+      unsigned long long mm = m0 + cc;
+      cd v = base[(unsigned long long)row * a.M + mm];
+      if (a.M > 1) {
+        unsigned long long k = freq_of_row<L>((unsigned)row);
+        v = cmulc(v, omega_pow(a.roots, mm * k * a.tw_mul));
+      }
+      lds[pad16(e)] = v;
     }
-    lds[pad16(e)] = v;
+  } else {
+    // conjugate pass twiddle, rows r0 + h * L/16 as in k_fwd
+    Walk16 w;
+    if (a.M > 1) w.init(a.roots, a.tw_mul * m * 16ull * brev((unsigned)r0, TC::LOG - 4), a.tw_mul * m);
+#pragma unroll
+    for (int h = 0; h < 16; h++) {
+      const int row = r0 + h * (L / 16);
+      cd v = base[(unsigned long long)row * a.M + m];
+      if (a.M > 1) v = cmulc(v, w.at(h));
+      lds[pad16(row * C + c)] = v;
+    }
   }
   __syncthreads();
   inverse_front<L, C>(lds, a.tl, tid);
@@ -219,11 +314,11 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
   constexpr int PER0 = TC::EPT / R0;
   constexpr int SUB0 = L / R0;
   double worst = 0.0;
+  cd wun = make_double2(1.0, 0.0);
+  if (UNPACK) wun = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
 #pragma unroll
   for (int i = 0; i < PER0; i++) {
-    int bf = tid + i * TC::T;
-    int c = bf % C;
-    int q = bf / C;
+    int q = r0 + i * (TC::T / C);
     cd x[R0];
 #pragma unroll
     for (int t = 0; t < R0; t++) x[t] = lds[pad16((q + t * SUB0) * C + c)];
@@ -231,14 +326,13 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
 #pragma unroll
     for (int t = 0; t < R0; t++) {
       unsigned long long row = (unsigned long long)(q + t * SUB0);
-      unsigned long long j = row * a.M + m0 + c;
       if (UNPACK)
-        worst = fmax(worst, unpack_point(a, x[t], j));
+        worst = fmax(worst, round_store(a, cmulc(x[t], cmul(wun, w64(i + t * PER0))), row, m, L));
       else
-        base[j] = x[t];
+        base[row * a.M + m] = x[t];
     }
   }
-  if (UNPACK) margin_reduce(a.margin, worst);
+  if (UNPACK) margin_reduce(a.margin, worst, a.margin_mask);
 }
 
 // ------------------------------------------------------------- middle pass
@@ -420,7 +514,7 @@ __global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
         a.data[j] = xa[i * R0 + t];
     }
   }
-  if (UNPACK) margin_reduce(a.margin, worst);
+  if (UNPACK) margin_reduce(a.margin, worst, a.margin_mask);
 }
 
 // ------------------------------------------------------------ carry resolve
@@ -456,6 +550,7 @@ struct CarryArgs {
   unsigned long long addl;
   int coef_T;                // coefficient j at coef[(j % cM) * crows + j / cM]
   unsigned cM, crows;
+  unsigned lds_slots;        // int64 slots of dynamic LDS per block (coefficient window)
 };
 
 __device__ __forceinline__ long long coef_at(const CarryArgs &a, long long j) {
@@ -514,24 +609,41 @@ __device__ __forceinline__ void bind_carry(CarryArgs &a, unsigned long long p) {
 }
 
 // Coefficient window of one carry block in LDS: words [W0, W0 + CT*CWPT) need
-// coefficients j with b j in (32 (W0-1) - cbits, 32 (W0 + CT*CWPT)).
-constexpr int CARRY_LDS = 8256;  // int64 slots: enough for b >= 4 and cbits <= 63
+// coefficients j with b j in (32 (W0-1) - cbits, 32 (W0 + CT*CWPT)); the
+// window is sized per launch (dynamic LDS) from the smallest digit size.
+// 0 (no staging: coefficients are read from global memory) when the window
+// would exceed CARRY_LDS_MAX slots (digit sizes below 4 bits).
+constexpr unsigned CARRY_LDS_MAX = 8256;
+__host__ __device__ inline unsigned carry_slots(int bits, int cbits) {
+  unsigned n = (unsigned)((32ull * CT * CWPT + (unsigned long long)cbits + 64) / (unsigned long long)bits + 4);
+  return n <= CARRY_LDS_MAX ? n : 0u;
+}
 
+// floor(x / b) for 0 <= x < 2^40 through a double reciprocal plus one fix-up.
+__device__ __forceinline__ long long div_small(long long x, int b, double inv_b) {
+  long long q = (long long)((double)x * inv_b);
+  if ((q + 1) * b <= x) q++;
+  if (q * b > x) q--;
+  return q;
+}
+
+// P_w of one word from the LDS window cl[] (coefficient jbase at cl[0]).
 __device__ __forceinline__ unsigned long long partial_from_lds(const long long *cl, long long jbase, long long ncoef,
-                                                               int bits, int cbits, long long w) {
+                                                               int bits, int cbits, double inv_b, long long w) {
   if (w < 0) return 0ull;
   long long lo_bit = 32 * w - cbits + 1;
-  long long jlo = lo_bit <= 0 ? 0 : (lo_bit + bits - 1) / bits;
-  long long jhi = (32 * w + 31) / bits;
+  long long jlo = lo_bit <= 0 ? 0 : div_small(lo_bit + bits - 1, bits, inv_b);
+  long long jhi = div_small(32 * w + 31, bits, inv_b);
   if (jhi >= ncoef) jhi = ncoef - 1;
   unsigned long long P = 0;
-  for (long long j = jlo; j <= jhi; j++) {
-    unsigned long long c = (unsigned long long)cl[j - jbase];
-    long long sh = (long long)bits * j - 32 * w;
+  int sh = (int)((long long)bits * jlo - 32 * w);
+  const long long *c = cl + (jlo - jbase);
+  for (long long j = jlo; j <= jhi; j++, sh += bits, c++) {
+    const unsigned long long v = (unsigned long long)*c;
     if (sh >= 0)
-      P += (unsigned long long)(uint32_t)(c << sh);
-    else if (-sh < 64)
-      P += (unsigned long long)(uint32_t)(c >> (-sh));
+      P += (unsigned long long)(uint32_t)(v << sh);
+    else if (sh > -64)
+      P += (unsigned long long)(uint32_t)(v >> (-sh));
   }
   return P;
 }
@@ -539,32 +651,54 @@ __device__ __forceinline__ unsigned long long partial_from_lds(const long long *
 // Words of one block from coefficients: P_w (< 2^37) per word, v_w = lo(P_w) +
 // hi(P_{w-1}), a wave-shuffle scan of per-thread carry functions, words
 // written for block carry-in 0 and the block's function saved for k_carry_fix.
-__global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
-  __shared__ long long cl[CARRY_LDS];
+// The coefficient window is staged into LDS with eight loads in flight per
+// thread; each thread's word w-1 partial comes from its left neighbour.
+static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
+  extern __shared__ long long cl[];
   __shared__ unsigned wave_fn[CT / 64];
   const int tid = threadIdx.x;
   bind_carry(a, blockIdx.y);
   if (blockIdx.x >= a.nblk) return;
   const long long W0 = (long long)blockIdx.x * CT * CWPT;
   const long long wfirst = W0 + (long long)tid * CWPT;
-  // stage the block's coefficient window in LDS (coalesced)
+  const double inv_b = 1.0 / (double)a.bits;
   long long lo_bit = 32 * (W0 - 1) - a.cbits + 1;
-  long long jbase = lo_bit <= 0 ? 0 : (lo_bit + a.bits - 1) / a.bits;
-  long long jend = (32 * (W0 + (long long)CT * CWPT) - 1) / a.bits;  // inclusive
+  long long jbase = lo_bit <= 0 ? 0 : div_small(lo_bit + a.bits - 1, a.bits, inv_b);
+  long long jend = div_small(32 * (W0 + (long long)CT * CWPT) - 1, a.bits, inv_b);  // inclusive
   if (jend >= (long long)a.ncoef) jend = (long long)a.ncoef - 1;
   const long long cnt = jend >= jbase ? jend - jbase + 1 : 0;
-  const bool staged = cnt <= CARRY_LDS;
-  if (staged)
-    for (long long t = tid; t < cnt; t += CT) cl[t] = coef_at(a, jbase + t);
+  const bool staged = cnt <= (long long)a.lds_slots;
+  if (staged) {
+    for (long long t0 = 0; t0 < cnt; t0 += 8 * CT) {
+      long long r[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const long long t = t0 + u * CT + tid;
+        r[u] = t < cnt ? coef_at(a, jbase + t) : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const long long t = t0 + u * CT + tid;
+        if (t < cnt) cl[t] = r[u];
+      }
+    }
+  }
   __syncthreads();
   unsigned long long P[CWPT + 1];
   if (staged) {
+    const int lane = tid & 63;
 #pragma unroll
-    for (int k = 0; k <= CWPT; k++) {
+    for (int k = 1; k <= CWPT; k++) {
       long long w = wfirst - 1 + k;
-      P[k] = (w >= 0 && w < (long long)a.CL) ? partial_from_lds(cl, jbase, (long long)a.ncoef, a.bits, a.cbits, w)
-                                            : 0ull;
+      P[k] = (w < (long long)a.CL) ? partial_from_lds(cl, jbase, (long long)a.ncoef, a.bits, a.cbits, inv_b, w) : 0ull;
     }
+    // word wfirst-1 is the left neighbour's last word
+    unsigned long long left = __shfl_up(P[CWPT], 1, 64);
+    if (lane == 0)
+      left = (wfirst - 1 < (long long)a.CL)
+                 ? partial_from_lds(cl, jbase, (long long)a.ncoef, a.bits, a.cbits, inv_b, wfirst - 1)
+                 : 0ull;
+    P[0] = left;
     if (a.add) {
 #pragma unroll
       for (int k = 0; k <= CWPT; k++) {
@@ -608,18 +742,25 @@ __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
     a.blk_fn[blockIdx.x] = (unsigned char)tot;
   }
   unsigned cin = carry_compose(pre, excl) & 1u;
+  uint32_t o[CWPT];
 #pragma unroll
   for (int k = 0; k < CWPT; k++) {
     unsigned long long sum = v[k] + cin;
-    long long w = wfirst + k;
-    if (w < (long long)a.CL) a.out[w] = (uint32_t)sum;
+    o[k] = (uint32_t)sum;
     cin = (unsigned)(sum >> 32);
+  }
+  if (wfirst + CWPT <= (long long)a.CL && ((reinterpret_cast<uintptr_t>(a.out + wfirst) & 15) == 0)) {
+    *reinterpret_cast<uint4 *>(a.out + wfirst) = make_uint4(o[0], o[1], o[2], o[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < CWPT; k++)
+      if (wfirst + k < (long long)a.CL) a.out[wfirst + k] = o[k];
   }
 }
 
 // One workgroup: scan the block carry functions, then add the carry into every
 // block whose carry-in is 1 (ripple through 0xffffffff words inside the block).
-__global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
+static __global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
   __shared__ unsigned char fns[1024];
   const int tid = threadIdx.x;
   bind_carry(a, blockIdx.x);
@@ -664,12 +805,12 @@ __global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
 
 // --------------------------------------------------------------- utilities
 
-__global__ void k_zero_u32(unsigned int *p, int n) {
+static __global__ void k_zero_u32(unsigned int *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
 }
 
-__global__ void k_fill_u32(uint32_t *p, unsigned long long n, uint32_t v) {
+static __global__ void k_fill_u32(uint32_t *p, unsigned long long n, uint32_t v) {
   unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long st = (unsigned long long)gridDim.x * blockDim.x;
   for (; i < n; i += st) p[i] = v;
@@ -838,7 +979,7 @@ namespace bcg {
 // Batched schoolbook products (CL < 100 words, AL + BL <= 128): one wave per
 // product; lane w sums column w of 32x32-bit partial products (low and high
 // halves apart), lane 0 then runs the carry chain, adding the optional addend.
-__global__ __launch_bounds__(64) void k_basecase_batch(const MulDesc *desc, int count) {
+static __global__ __launch_bounds__(64) void k_basecase_batch(const MulDesc *desc, int count) {
   __shared__ unsigned long long lo[128], hi[128];
   const int p = blockIdx.x;
   if (p >= count) return;

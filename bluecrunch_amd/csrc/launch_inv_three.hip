@@ -1,0 +1,17 @@
+// Inverse pass kernels, 3 * 2^k tiles.
+#include "launch_impl.h"
+
+namespace bcg {
+fwd_fn pick_inv_three(int L, int C) {
+  switch (L) {
+    case 96: return ByC<96, InvL>::pick(C);
+    case 192: return ByC<192, InvL>::pick(C);
+    case 384: return ByC<384, InvL>::pick(C);
+    case 768: return ByC<768, InvL>::pick(C);
+    case 1536: return ByC<1536, InvL>::pick(C);
+    case 3072: return ByC<3072, InvL>::pick(C);
+    case 6144: return ByC<6144, InvL>::pick(C);
+  }
+  return nullptr;
+}
+}  // namespace bcg

@@ -51,6 +51,9 @@ int bcg_malloc(void **ptr, size_t bytes);
 int bcg_free(void *ptr);
 int bcg_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int bcg_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain vector copy kernel on the library stream
+ * (bytes a multiple of 4); also the yardstick for HBM traffic counters. */
+int bcg_memcpy_d2d(void *dst, const void *src, size_t bytes);
 
 /* C = low CL words of A*B (little-endian base 2^32 words). A == B with
  * AL == BL selects the squaring path. C may alias A or B. Replaces the
@@ -118,6 +121,9 @@ int bcg_plan_candidates(size_t N, int index, int *passes, int *L, int *C);
 /* Use candidate `index` for every transform of N points (-1: back to the
  * default or the autotuned choice). */
 int bcg_plan_force(size_t N, int index);
+/* Index of the candidate transforms of N points use now (forced, autotuned
+ * or default); negative BCG_E* code when N has no split. */
+int bcg_plan_current(size_t N);
 /* Autotuner: times every pass split of the transform an AL x BL product uses
  * (synthetic words, `reps` runs each) and keeps the fastest for that length.
  * times_ms (optional) receives up to max_out per-candidate milliseconds. */

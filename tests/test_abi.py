@@ -67,6 +67,26 @@ def test_plan_at_benchmark_size():
     assert p["bits"] == 8 and p["points"] == 1 << 26
 
 
+def test_default_split_keeps_strided_passes_wide():
+    # the untuned split: every strided (non-innermost) pass moves >= 8 columns
+    # per workgroup, the last pass is contiguous; forcing and resetting works
+    for w in [1 << 16, 1 << 20, 1 << 22, 1 << 24, 3 << 21]:
+        n = bc.plan(w, w)["points"]
+        cands = bc.plan_candidates(n)
+        cur = bc.plan_current(n)
+        P, L, C = cands[cur]
+        assert len(L) == P and all(c >= 8 for c in C[:-1]), (w, cands[cur])
+        prod = 1
+        for x in L:
+            prod *= x
+        assert prod == n
+        other = (cur + 1) % len(cands)
+        bc.plan_force(n, other)
+        assert bc.plan_current(n) == other
+        bc.plan_force(n, -1)
+        assert bc.plan_current(n) == cur
+
+
 def test_no_gpu_fails_loudly():
     # without a HIP device the product path returns an error code, never a CPU result
     import ctypes

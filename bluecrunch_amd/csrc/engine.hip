@@ -44,8 +44,7 @@ struct Engine {
   hipStream_t stream = nullptr;
   std::map<unsigned long long, RootsDev> roots;  // keyed by N
   std::map<int, cd *> ltab;                      // e^{2 pi i t/L}, keyed by L
-  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, margins, counters, bbufA, bbufB, bblk, bdesc, tdesc,
-      digA, digB;
+  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, margins, counters, bbufA, bbufB, bblk, bdesc, tdesc;
   bool prof = false;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> free_events;
@@ -319,10 +318,26 @@ bool fill_strides(Plan &p) {
 
 bool is_pow2(unsigned long long v) { return v && !(v & (v - 1)); }
 
+// Adds p and, for each strided pass with 16-column tiles, the variant with
+// 8-column tiles (half the LDS per workgroup: more workgroups per CU).
+void push_with_tiles(std::vector<Plan> &out, const Plan &p) {
+  const int strided = p.P - 1;
+  for (int mask = 0; mask < (1 << strided); mask++) {
+    Plan q = p;
+    bool ok = true;
+    for (int i = 0; i < strided; i++)
+      if (mask & (1 << i)) {
+        if (q.C[i] != 16) ok = false;
+        q.C[i] = 8;
+      }
+    if (ok) out.push_back(q);
+  }
+}
+
 // Every pass split of N the kernels support: the innermost (contiguous) pass
 // carries the 3*2^k factor and is <= 8192 (<= 6144); outer passes are powers
 // of two in [64, 8192]; at most three passes. Ordered by decreasing inner
-// pass, two-pass before three-pass.
+// pass, two-pass before three-pass; each split first with its widest tiles.
 std::vector<Plan> factor_candidates(unsigned long long N) {
   std::vector<Plan> out;
   const bool three = is_three(N);
@@ -346,7 +361,7 @@ std::vector<Plan> factor_candidates(unsigned long long N) {
       p.P = 2;
       p.L[0] = (int)rest;
       p.L[1] = (int)inner;
-      if (fill_strides(p)) out.push_back(p);
+      if (fill_strides(p)) push_with_tiles(out, p);
     }
     for (unsigned long long l1 = 64; l1 <= 8192; l1 *= 2) {
       unsigned long long l2 = rest / l1;
@@ -357,7 +372,7 @@ std::vector<Plan> factor_candidates(unsigned long long N) {
       p.L[0] = (int)l1;
       p.L[1] = (int)l2;
       p.L[2] = (int)inner;
-      if (fill_strides(p)) out.push_back(p);
+      if (fill_strides(p)) push_with_tiles(out, p);
     }
   }
   return out;
@@ -457,8 +472,11 @@ int coef_bits(const Plan &p) {
   return cb > 63 ? 63 : cb;
 }
 
-// Fill the common pass arguments for pass i of plan p.
+// Fill the common pass arguments for pass i of plan p. The packing pass
+// (i = 0 of a multi-pass plan) slices digits assuming M b and N b are
+// multiples of 32.
 int pass_args(const Plan &p, int i, PassArgs &a) {
+  if (p.P > 1 && i == 0 && (p.M[0] % 32 || p.N % 32)) return fail(BCG_E_ARG, "packing pass needs 32 | M and 32 | N");
   a.N = p.N;
   a.M = p.M[i];
   a.S = (unsigned long long)p.L[i] * p.M[i];
@@ -504,6 +522,45 @@ int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbi
   return 0;
 }
 
+// Carry resolve of plan p's last pass output (group sums or coefficients in
+// bufB) into CL words (+ addend D).
+int group_words(const Plan &p);
+int group_stride(int K);
+bool coef_transposed(const Plan &p);
+int carry_of_plan(const Plan &p, uint32_t *out, unsigned long long CL, hipStream_t s, const uint32_t *add,
+                  unsigned long long addl) {
+  const int K = group_words(p);
+  if (!K)
+    return run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), out, CL, s, add, addl,
+                     coef_transposed(p) ? (unsigned)p.M[0] : 0u, 2u * (unsigned)p.L[0]);
+  if (CL == 0) return 0;
+  const unsigned long long wpb = (unsigned long long)CT * CWPT;
+  unsigned long long nblk = (CL + wpb - 1) / wpb;
+  if (int rc = ensure(g.blk, nblk)) return rc;
+  CarryArgs c;
+  memset(&c, 0, sizeof c);
+  c.ncoef = 2 * p.N;
+  c.bits = p.bits;
+  c.cbits = coef_bits(p);
+  c.out = out;
+  c.CL = CL;
+  c.blk_fn = (unsigned char *)g.blk.p;
+  c.nblk = nblk;
+  c.add = add;
+  c.addl = addl;
+  c.ybuf = (const uint32_t *)g.bufB.p;
+  c.gC = p.C[0];
+  c.gK = K;
+  c.gKs = group_stride(K);
+  {
+    ProfScope ps(BCG_K_CARRY, s);
+    hipLaunchKernelGGL(k_carry_local, dim3((unsigned)nblk), dim3(CT), 0, s, c);
+    LAUNCH(k_carry_fix, 1, 1024, s, c);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int zero_margin(hipStream_t s) {
   const int n = MARGIN_SLOTS * MARGIN_STRIDE;
   LAUNCH(k_zero_u32, (n + 255) / 256, 256, s, (unsigned int *)g.margin.p, n);
@@ -522,30 +579,25 @@ int read_margin(hipStream_t s, double *m) {
   return 0;
 }
 
-// 1: the first pass reads digits from a transposed copy made by k_digits_T;
-// 0: it slices them straight out of the operand words.
-#ifndef BCG_STAGE_DIGITS
-#define BCG_STAGE_DIGITS 0
-#endif
-
-int digit_bytes(int bits) { return bits <= 8 ? 1 : (bits <= 16 ? 2 : 4); }
-
-// Operand digits, transposed for the first pass of plan p (see k_digits_T).
-int stage_digits(const Plan &p, const uint32_t *W, unsigned long long WL, DevBuf &dst, hipStream_t s) {
-  const unsigned long long rows = 2ull * (unsigned long long)p.L[0], M = p.M[0];
-  const int ds = digit_bytes(p.bits);
-  if (M % 64 || rows % 64) return fail(BCG_E_ARG, "digit staging needs 64 | M and 64 | 2L");
-  if (int rc = ensure(dst, (size_t)(rows * M) * ds)) return rc;
-  dim3 grid((unsigned)(M / 64), (unsigned)(rows / 64));
-  ProfScope ps(BCG_K_OTHER, s);
-  if (ds == 1)
-    hipLaunchKernelGGL(k_digits_T<uint8_t>, grid, dim3(256), 0, s, W, WL, p.bits, rows, M, (uint8_t *)dst.p);
-  else if (ds == 2)
-    hipLaunchKernelGGL(k_digits_T<uint16_t>, grid, dim3(256), 0, s, W, WL, p.bits, rows, M, (uint16_t *)dst.p);
-  else
-    hipLaunchKernelGGL(k_digits_T<uint32_t>, grid, dim3(256), 0, s, W, WL, p.bits, rows, M, (uint32_t *)dst.p);
-  HIPCHK(hipGetLastError());
-  return 0;
+// Group mode: the last inverse pass of a multi-pass plan with 8- or
+// 16-column tiles writes, per row segment of C digits, the exact sum
+// sum_i c_i 2^{b i} as K words instead of C 64-bit coefficients (about a
+// quarter of the bytes), and the carry reads those sums. Returns K, or 0 when
+// the plan keeps plain coefficients; group_stride rounds K up to 4 words.
+int group_stride(int K) { return (K + 3) & ~3; }
+int group_words(const Plan &p) {
+  if (p.P < 2 || (p.C[0] != 8 && p.C[0] != 16)) return 0;
+  const int G = p.C[0], b = p.bits, cb = coef_bits(p);
+  int off_max = 0;
+  for (int g = 0; g < 32; g++) {
+    int o = (G * b * g) & 31;
+    off_max = o > off_max ? o : off_max;
+  }
+  const int K = (off_max + b * (G - 1) + cb + 31) / 32;
+  // groups covering one word: at most ceil(32 K / (G b)) + 1
+  const int cover = (32 * K + G * b - 1) / (G * b) + 1;
+  if (cover > CARRY_GCOV || group_stride(K) > 8) return 0;
+  return K;
 }
 
 // Coefficients of plan p are written transposed when the last pass has
@@ -582,22 +634,15 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     HIPCHK(hipGetLastError());
     return 0;
   }
-  // forward outer passes for A (and B); digits staged transposed first
+  // forward outer passes for A (and B)
   for (int op = 0; op < (square ? 1 : 2); op++) {
     cd *buf = op == 0 ? da : db;
-    DevBuf &dig = op == 0 ? g.digA : g.digB;
-    if (BCG_STAGE_DIGITS)
-      if (int rc = stage_digits(p, op == 0 ? A : B, op == 0 ? AL : BL, dig, s)) return rc;
     for (int i = 0; i < p.P - 1; i++) {
       PassArgs b = a;
       if (int rc = pass_args(p, i, b)) return rc;
       b.data = buf;
       b.wa = op == 0 ? A : B;
       b.al = op == 0 ? AL : BL;
-      if (i == 0 && BCG_STAGE_DIGITS) {
-        b.dig = dig.p;
-        b.dig_size = digit_bytes(p.bits);
-      }
       fwd_fn f = pick_fwd(p.L[i], p.C[i]);
       if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
       ProfScope ps(i == 0 ? BCG_K_FWD_PACK : BCG_K_FWD, s);
@@ -625,6 +670,10 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     b.data = da;
     b.coef = coef;
     b.coef_T = (i == 0 && coef_transposed(p)) ? 1 : 0;
+    if (i == 0 && group_words(p)) {
+      b.ybuf = (uint32_t *)g.bufB.p;
+      b.ygK = group_stride(group_words(p));
+    }
     fwd_fn f = pick_inv(p.L[i], p.C[i]);
     if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
     ProfScope ps(i == 0 ? BCG_K_INV_UNPACK : BCG_K_INV, s);
@@ -904,9 +953,7 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
       if (int rc = read_margin(s, &m)) return rc;
     }
     if (!alias) {
-      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s, D, DL,
-                             coef_transposed(p) ? (unsigned)p.M[0] : 0u, 2u * (unsigned)p.L[0]))
-        return rc;
+      if (int rc = carry_of_plan(p, C, CL, s, D, DL)) return rc;
       if (int rc = read_margin(s, &m)) return rc;
     }
     inf.bits = p.bits;
@@ -925,9 +972,7 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
       continue;
     }
     if (alias) {
-      if (int rc = run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), C, CL, s, D, DL,
-                             coef_transposed(p) ? (unsigned)p.M[0] : 0u, 2u * (unsigned)p.L[0]))
-        return rc;
+      if (int rc = carry_of_plan(p, C, CL, s, D, DL)) return rc;
     }
     break;
   }
@@ -1191,7 +1236,7 @@ int bcg_shutdown(void) {
   if (!g.ready) return 0;
   (void)hipStreamSynchronize(g.stream);
   for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf, &g.desc, &g.margins,
-                    &g.counters, &g.bbufA, &g.bbufB, &g.bblk, &g.bdesc, &g.tdesc, &g.digA, &g.digB}) {
+                    &g.counters, &g.bbufA, &g.bbufB, &g.bblk, &g.bdesc, &g.tdesc}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->bytes = 0;
@@ -1714,7 +1759,9 @@ int bcg_plan_current(size_t N) {
   auto it = g_cands.find(N);
   for (size_t i = 0; i < it->second.size(); i++) {
     const Plan &c = it->second[i];
-    if (c.P == p.P && c.L[0] == p.L[0] && c.L[1] == p.L[1] && c.L[2] == p.L[2]) return (int)i;
+    if (c.P == p.P && c.L[0] == p.L[0] && c.L[1] == p.L[1] && c.L[2] == p.L[2] && c.C[0] == p.C[0] &&
+        c.C[1] == p.C[1])
+      return (int)i;
   }
   return fail(BCG_E_ARG, "no pass split for this length");
 }

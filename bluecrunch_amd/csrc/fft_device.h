@@ -184,6 +184,22 @@ struct Walk16 {
     g4 = omega_direct(r, 4 * du);
     g8 = omega_direct(r, 8 * du);
   }
+  // Same, when lanes l, l + C, l + 2C, l + 3C of each 4C-lane block share du
+  // (same column): each computes one of g, g^2, g^4, g^8 and the four are
+  // exchanged by lane shuffles. Needs 4C <= 64 and all 4C lanes active.
+  template <int C>
+  __device__ __forceinline__ void init_shared(const Roots &r, unsigned long long u0, unsigned long long du) {
+    static_assert(4 * C <= 64, "four lanes per column within a wave");
+    const int lane = threadIdx.x & 63;
+    const int q = (lane / C) & 3;
+    const int src0 = (lane % C) + (lane / (4 * C)) * (4 * C);
+    base = omega_direct(r, u0);
+    const cd mine = omega_direct(r, du << q);
+    g1 = make_double2(__shfl(mine.x, src0, 64), __shfl(mine.y, src0, 64));
+    g2 = make_double2(__shfl(mine.x, src0 + C, 64), __shfl(mine.y, src0 + C, 64));
+    g4 = make_double2(__shfl(mine.x, src0 + 2 * C, 64), __shfl(mine.y, src0 + 2 * C, 64));
+    g8 = make_double2(__shfl(mine.x, src0 + 3 * C, 64), __shfl(mine.y, src0 + 3 * C, 64));
+  }
   __device__ __forceinline__ cd at(int i) {
     if ((i & 7) == 0) p3 = (i & 8) ? cmul(base, g1) : base;
     if ((i & 3) == 0) p2 = (i & 4) ? cmul(p3, g2) : p3;

@@ -37,9 +37,9 @@ struct PassArgs {
   double inv_n;
   const MulDesc *desc;      // batched launch: product blockIdx.y, else nullptr
   int operand;              // batched first pass: 0 reads desc.a, 1 reads desc.b
-  const void *dig;          // first pass: digits transposed, dig[m * 2L + t] = digit t*M + m
-  int dig_size;             // bytes per transposed digit (1, 2 or 4)
   int coef_T;               // last pass: coefficient j at coef[(j % M) * 2L + j / M]
+  uint32_t *ybuf;           // last pass, group mode: digits [C g, C g + C) summed into ybuf[g K .. g K + K)
+  int ygK;                  // words per group sum (a multiple of 4: 16-byte stores)
 };
 
 // Batched launches (desc != nullptr): product p = blockIdx.y owns the arrays
@@ -91,20 +91,6 @@ __device__ __forceinline__ cd packed_point(const PassArgs &a, const uint32_t *__
   return cmul(packed_digits(a, w, wl, j), omega_pow(a.roots, j));
 }
 
-// Right-angle input point (row t, column m) of the first pass from transposed
-// digits: a_j = dig[m * 2L + t], a_{j+N} = dig[m * 2L + L + t], j = t M + m.
-__device__ __forceinline__ double dig_load(const PassArgs &a, unsigned long long idx) {
-  if (a.dig_size == 1) return (double)((const uint8_t *)a.dig)[idx];
-  if (a.dig_size == 2) return (double)((const uint16_t *)a.dig)[idx];
-  return (double)((const uint32_t *)a.dig)[idx];
-}
-
-__device__ __forceinline__ cd packed_digits_T(const PassArgs &a, unsigned long long row, unsigned long long m,
-                                              unsigned long long L) {
-  unsigned long long idx = m * 2 * L + row;
-  return make_double2(dig_load(a, idx), dig_load(a, idx + L));
-}
-
 __device__ __forceinline__ void margin_update(unsigned int *m, double d) {
   float f = (float)d;
   // non-negative floats order like their bit patterns
@@ -132,6 +118,67 @@ __device__ __forceinline__ double round_store(const PassArgs &a, cd f, unsigned 
     a.coef[j + a.N] = ci > 0 ? ci : 0;
   }
   return m;
+}
+
+// Rounded coefficient pair of f = w_j Omega^{-j} (unscaled); updates worst.
+__device__ __forceinline__ void round_pair(const PassArgs &a, cd f, long long &cr, long long &ci, double &worst) {
+  double re = f.x * a.inv_n, im = f.y * a.inv_n;
+  worst = fmax(worst, fmax(fabs(re - rint(re)), fabs(im - rint(im))));
+  cr = (long long)(re + 0.5);
+  ci = (long long)(im + 0.5);
+  cr = cr > 0 ? cr : 0;
+  ci = ci > 0 ? ci : 0;
+}
+
+// Group sums of the last inverse pass (group mode), one half (h = 0: real
+// parts, digits j < N; h = 1: imaginary parts, j >= N) at a time: row t of
+// the tile holds the C coefficients of digits j0 + [0, C), j0 = t M + m0 + h N,
+// at cl[t (C + 1) ...] (one pad slot per row against LDS bank conflicts).
+// Their sum sum_i c_i 2^{b i} is written exactly, shifted by (j0 b mod 32),
+// as ygK little-endian words (ygK a multiple of 4, zero padded) at
+// ybuf[(j0 / C) ygK ...]: the words from word floor(j0 b / 32) up. Words are
+// emitted into the row's own LDS slots (already consumed) and leave as
+// 16-byte stores.
+template <int L, int C>
+__device__ __forceinline__ void write_group_sums(const PassArgs &a, long long *cl, unsigned long long m0, int half,
+                                                 int tid) {
+  using TC = Tile<L, C>;
+  const int b = a.bits;
+  const int K = a.ygK;
+  for (int t = tid; t < L; t += TC::T) {
+    const unsigned long long j0 = (unsigned long long)t * a.M + m0 + (half ? a.N : 0ull);
+    const long long *src = cl + t * (C + 1);
+    uint32_t *lw = reinterpret_cast<uint32_t *>(cl + t * (C + 1));
+    // 128-bit accumulator (hi:lo); bit 0 is bit 32 k of the group sum.
+    unsigned long long lo = 0, hi = 0;
+    int pos = (int)((j0 * (unsigned long long)b) & 31ull);
+    int k = 0;
+    unsigned long long cv[C];  // all loads before the first (aliasing) word store
+#pragma unroll
+    for (int i = 0; i < C; i++) cv[i] = (unsigned long long)src[i];
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+      const unsigned long long v = cv[i];
+      if (pos >= 32) {
+        lw[k++] = (uint32_t)lo;
+        lo = (lo >> 32) | (hi << 32);
+        hi >>= 32;
+        pos -= 32;
+      }
+      const unsigned long long add_lo = v << pos;
+      const unsigned long long add_hi = pos ? (v >> (64 - pos)) : 0ull;
+      lo += add_lo;
+      hi += add_hi + (lo < add_lo ? 1ull : 0ull);
+      pos += b;
+    }
+    for (; k < K; k++) {
+      lw[k] = (uint32_t)lo;
+      lo = (lo >> 32) | (hi << 32);
+      hi >>= 32;
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(a.ybuf + (j0 / C) * (unsigned long long)K);
+    for (int q = 0; q < K / 4; q++) dst[q] = make_uint4(lw[4 * q], lw[4 * q + 1], lw[4 * q + 2], lw[4 * q + 3]);
+  }
 }
 
 // Whole-point form for single-pass plans (M = 1): j = row.
@@ -182,42 +229,56 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   cd wpack = make_double2(1.0, 0.0);
   // right-angle weight Omega^{row M + m} = Omega^{r0 M + m} e^{2 pi i h / 64}
   if (PACK) wpack = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
-  // PACK without a transposed digit copy: the words under each of the 2L row
-  // segments (row t, half h: digits t M + m0 + h N + [0, C)) are staged in LDS
-  // with coalesced loads, then every thread slices its 32 digits out of LDS.
+  // PACK: the words under each of the 2L row segments (row t, half h: digits
+  // t M + m0 + h N + [0, C)) are staged in LDS with coalesced loads, then every
+  // thread slices its 32 digits out of LDS. M b and N b are multiples of 32
+  // (M >= 128, N >= 8192 in multi-pass plans), so a digit's bit offset inside
+  // its segment depends only on its column.
   uint32_t dre[PACK ? 16 : 1], dim_[PACK ? 16 : 1];
   if constexpr (PACK) {
-    if (!a.dig) {
-      uint32_t *lw = reinterpret_cast<uint32_t *>(lds);
-      const int b = a.bits;
-      const unsigned SW = (unsigned)(C * b + 62) >> 5;  // words spanned by C digits
-      const unsigned inv = 0xffffffffu / SW + 1u;        // ceil(2^32 / SW)
-      const unsigned n = 2u * L * SW;
-      for (unsigned e = tid; e < n; e += TC::T) {
+    uint32_t *lw = reinterpret_cast<uint32_t *>(lds);
+    const unsigned b = (unsigned)a.bits;
+    const unsigned SW = (C * b + 62) >> 5;                // words spanned by C digits
+    const unsigned inv = 0xffffffffu / SW + 1u;            // ceil(2^32 / SW)
+    const unsigned long long row_w = (a.M * b) >> 5;       // words per row of digits
+    const unsigned long long half_w = (a.N * b) >> 5;      // words per half
+    const unsigned long long w0 = (m0 * b) >> 5;
+    const bool upper = half_w < a.al;                      // does the upper half hold digits?
+    const unsigned n = (upper ? 2u : 1u) * L * SW;
+    // eight independent loads in flight per thread per batch
+    for (unsigned e0 = 0; e0 < n; e0 += 8u * TC::T) {
+      uint32_t r[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const unsigned e = e0 + (unsigned)u * TC::T + (unsigned)tid;
         const unsigned seg = __umulhi(e, inv), k = e - seg * SW;
-        const unsigned long long j0 = (unsigned long long)(seg & (L - 1)) * a.M + m0 + (seg >= (unsigned)L ? a.N : 0ull);
-        const unsigned long long wi = ((j0 * (unsigned long long)b) >> 5) + k;
-        lw[e] = wi < a.al ? a.wa[wi] : 0u;
+        const unsigned long long wi =
+            (unsigned long long)(seg & (L - 1)) * row_w + w0 + (seg >= (unsigned)L ? half_w : 0ull) + k;
+        r[u] = (e < n && wi < a.al) ? a.wa[wi] : 0u;
       }
-      __syncthreads();
-      const unsigned mask = (1u << b) - 1u;
 #pragma unroll
-      for (int h = 0; h < 16; h++) {
-        const unsigned row = (unsigned)r0 + (unsigned)h * (L / 16);
-#pragma unroll
-        for (int half = 0; half < 2; half++) {
-          const unsigned seg = row + (half ? (unsigned)L : 0u);
-          // bit offset of digit (row, m) inside its segment's first word (mod 2^32 arithmetic)
-          const unsigned j0lo = (unsigned)(row * (unsigned)a.M + (unsigned)m0 + (half ? (unsigned)a.N : 0u));
-          const unsigned off = ((j0lo * (unsigned)b) & 31u) + (unsigned)(c * b);
-          const unsigned k = seg * SW + (off >> 5);
-          const unsigned long long v = (unsigned long long)lw[k] | ((unsigned long long)lw[k + 1] << 32);
-          const uint32_t d = (uint32_t)(v >> (off & 31u)) & mask;
-          if (half) dim_[h] = d; else dre[h] = d;
-        }
+      for (int u = 0; u < 8; u++) {
+        const unsigned e = e0 + (unsigned)u * TC::T + (unsigned)tid;
+        if (e < n) lw[e] = r[u];
       }
-      __syncthreads();
     }
+    __syncthreads();
+    const unsigned mask = (1u << b) - 1u;
+    const unsigned off = (((unsigned)m0 * b) & 31u) + (unsigned)c * b;
+    const unsigned kk = off >> 5, sh = off & 31u;
+#pragma unroll
+    for (int h = 0; h < 16; h++) {
+      const unsigned row = (unsigned)r0 + (unsigned)h * (L / 16);
+      const unsigned k0 = row * SW + kk;
+      dre[h] = (uint32_t)((((unsigned long long)lw[k0 + 1] << 32) | lw[k0]) >> sh) & mask;
+      if (upper) {
+        const unsigned k1 = k0 + (unsigned)L * SW;
+        dim_[h] = (uint32_t)((((unsigned long long)lw[k1 + 1] << 32) | lw[k1]) >> sh) & mask;
+      } else {
+        dim_[h] = 0u;
+      }
+    }
+    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < PER0; i++) {
@@ -228,7 +289,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
       unsigned long long row = (unsigned long long)(q + t * SUB0);
       if constexpr (PACK) {
         const int h = i + t * PER0;
-        cd v = a.dig ? packed_digits_T(a, row, m, L) : make_double2((double)dre[h], (double)dim_[h]);
+        cd v = make_double2((double)dre[h], (double)dim_[h]);
         x[t] = cmul(v, cmul(wpack, w64(h)));
       } else {
         x[t] = base[row * a.M + m];
@@ -254,7 +315,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   } else {
     // pass twiddle e^{2 pi i m k / S} = Omega^{tw_mul m k}
     Walk16 w;
-    if (a.M > 1) w.init(a.roots, a.tw_mul * m * 16ull * brev((unsigned)r0, TC::LOG - 4), a.tw_mul * m);
+    if (a.M > 1) w.template init_shared<C>(a.roots, a.tw_mul * m * 16ull * brev((unsigned)r0, TC::LOG - 4), a.tw_mul * m);
 #pragma unroll
     for (int h = 0; h < 16; h++) {
       const int row = r0 + h * (L / 16);
@@ -298,7 +359,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
   } else {
     // conjugate pass twiddle, rows r0 + h * L/16 as in k_fwd
     Walk16 w;
-    if (a.M > 1) w.init(a.roots, a.tw_mul * m * 16ull * brev((unsigned)r0, TC::LOG - 4), a.tw_mul * m);
+    if (a.M > 1) w.template init_shared<C>(a.roots, a.tw_mul * m * 16ull * brev((unsigned)r0, TC::LOG - 4), a.tw_mul * m);
 #pragma unroll
     for (int h = 0; h < 16; h++) {
       const int row = r0 + h * (L / 16);
@@ -316,6 +377,8 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
   double worst = 0.0;
   cd wun = make_double2(1.0, 0.0);
   if (UNPACK) wun = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
+  const bool groups = UNPACK && a.ybuf;
+  long long cre[UNPACK ? 16 : 1], cim[UNPACK ? 16 : 1];
 #pragma unroll
   for (int i = 0; i < PER0; i++) {
     int q = r0 + i * (TC::T / C);
@@ -326,13 +389,37 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
 #pragma unroll
     for (int t = 0; t < R0; t++) {
       unsigned long long row = (unsigned long long)(q + t * SUB0);
-      if (UNPACK)
-        worst = fmax(worst, round_store(a, cmulc(x[t], cmul(wun, w64(i + t * PER0))), row, m, L));
-      else
+      if constexpr (UNPACK) {
+        const int h = i + t * PER0;
+        const cd f = cmulc(x[t], cmul(wun, w64(h)));
+        if (groups)
+          round_pair(a, f, cre[h], cim[h], worst);
+        else
+          worst = fmax(worst, round_store(a, f, row, m, L));
+      } else {
         base[row * a.M + m] = x[t];
+      }
     }
   }
-  if (UNPACK) margin_reduce(a.margin, worst, a.margin_mask);
+  if constexpr (UNPACK) {
+    if (groups) {
+      // coefficients through LDS (row segments of C), then one exact group
+      // sum per segment
+      long long *cl = reinterpret_cast<long long *>(lds);
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 16; h++) {
+          const int row = r0 + h * (L / 16);
+          cl[row * (C + 1) + c] = half ? cim[h] : cre[h];
+        }
+        __syncthreads();
+        write_group_sums<L, C>(a, cl, m0, half, tid);
+      }
+    }
+    margin_reduce(a.margin, worst, a.margin_mask);
+  }
 }
 
 // ------------------------------------------------------------- middle pass
@@ -551,6 +638,8 @@ struct CarryArgs {
   int coef_T;                // coefficient j at coef[(j % cM) * crows + j / cM]
   unsigned cM, crows;
   unsigned lds_slots;        // int64 slots of dynamic LDS per block (coefficient window)
+  const uint32_t *ybuf;      // group mode: group sums instead of coefficients (ncoef digits in all)
+  int gC, gK, gKs;           // digits per group, words per group sum, group stride (words)
 };
 
 __device__ __forceinline__ long long coef_at(const CarryArgs &a, long long j) {
@@ -627,6 +716,26 @@ __device__ __forceinline__ long long div_small(long long x, int b, double inv_b)
   return q;
 }
 
+// P_w from group sums (see write_group_sums): group g (digits [G g, G g + G))
+// covers words W_g .. W_g + K - 1, W_g = floor(G b g / 32); at most
+// CARRY_GCOV groups cover one word.
+constexpr int CARRY_GCOV = 4;
+__device__ __forceinline__ unsigned long long partial_from_groups(const CarryArgs &a, long long w, double inv_gb) {
+  if (w < 0) return 0ull;
+  const long long gb = (long long)a.gC * a.bits;
+  long long g = div_small(32 * w + 31, (int)gb, inv_gb);
+  const long long ng = (long long)(a.ncoef / (unsigned long long)a.gC);
+  if (g >= ng) g = ng - 1;
+  unsigned long long P = 0;
+#pragma unroll
+  for (int t = 0; t < CARRY_GCOV; t++) {
+    const long long gg = g - t;
+    const long long wg = (gg * gb) >> 5;
+    if (gg >= 0 && wg + a.gK > w && wg <= w) P += a.ybuf[gg * a.gKs + (w - wg)];
+  }
+  return P;
+}
+
 // P_w of one word from the LDS window cl[] (coefficient jbase at cl[0]).
 __device__ __forceinline__ unsigned long long partial_from_lds(const long long *cl, long long jbase, long long ncoef,
                                                                int bits, int cbits, double inv_b, long long w) {
@@ -666,8 +775,8 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
   long long jbase = lo_bit <= 0 ? 0 : div_small(lo_bit + a.bits - 1, a.bits, inv_b);
   long long jend = div_small(32 * (W0 + (long long)CT * CWPT) - 1, a.bits, inv_b);  // inclusive
   if (jend >= (long long)a.ncoef) jend = (long long)a.ncoef - 1;
-  const long long cnt = jend >= jbase ? jend - jbase + 1 : 0;
-  const bool staged = cnt <= (long long)a.lds_slots;
+  const long long cnt = a.ybuf ? 0 : (jend >= jbase ? jend - jbase + 1 : 0);
+  const bool staged = !a.ybuf && cnt <= (long long)a.lds_slots;
   if (staged) {
     for (long long t0 = 0; t0 < cnt; t0 += 8 * CT) {
       long long r[8];
@@ -685,7 +794,15 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
   }
   __syncthreads();
   unsigned long long P[CWPT + 1];
-  if (staged) {
+  if (a.ybuf) {
+    const double inv_gb = 1.0 / (double)(a.gC * a.bits);
+#pragma unroll
+    for (int k = 0; k <= CWPT; k++) {
+      long long w = wfirst - 1 + k;
+      P[k] = (w < (long long)a.CL) ? partial_from_groups(a, w, inv_gb) : 0ull;
+      if (a.add && w >= 0 && w < (long long)a.addl && w < (long long)a.CL) P[k] += a.add[w];
+    }
+  } else if (staged) {
     const int lane = tid & 63;
 #pragma unroll
     for (int k = 1; k <= CWPT; k++) {
@@ -1013,23 +1130,5 @@ static __global__ __launch_bounds__(64) void k_basecase_batch(const MulDesc *des
 }  // namespace bcg
 
 namespace bcg {
-
-// Digits of the operand words transposed for the first pass: the 2L x M
-// digit matrix (digit j = t M + m, row-major in j) written column-major,
-// out[m * rows + t], through 64x64 LDS tiles. Digits t*M + m past the words
-// are zero.
-template <typename DT>
-__global__ __launch_bounds__(256) void k_digits_T(const uint32_t *__restrict__ w, unsigned long long wl, int bits,
-                                                  unsigned long long rows, unsigned long long M, DT *__restrict__ out) {
-  __shared__ DT tile[64][65];
-  const unsigned long long m0 = (unsigned long long)blockIdx.x * 64, t0 = (unsigned long long)blockIdx.y * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int i = ty; i < 64; i += 4) {
-    unsigned long long j = (t0 + i) * M + m0 + tx;
-    tile[i][tx] = (DT)digit_at(w, wl, j, bits);
-  }
-  __syncthreads();
-  for (int i = ty; i < 64; i += 4) out[(m0 + i) * rows + t0 + tx] = tile[tx][i];
-}
 
 }  // namespace bcg

@@ -485,6 +485,7 @@ int pass_args(const Plan &p, int i, PassArgs &a) {
   if (int rc = get_roots(p.N, &a.roots)) return rc;
   a.bits = p.bits;
   a.inv_n = 1.0 / (double)p.N;
+  a.fast_round = coef_bits(p) <= 51 ? 1 : 0;
   a.ntiles = (int)(p.N / ((unsigned long long)p.L[i] * p.C[i]));
   return 0;
 }
@@ -496,7 +497,7 @@ int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbi
   if (CL == 0) return 0;
   const unsigned long long wpb = (unsigned long long)CT * CWPT;
   unsigned long long nblk = (CL + wpb - 1) / wpb;
-  if (int rc = ensure(g.blk, nblk)) return rc;
+  if (int rc = ensure(g.blk, (nblk + 15) & ~15ull)) return rc;
   CarryArgs c;
   memset(&c, 0, sizeof c);
   c.coef = coef;
@@ -536,7 +537,7 @@ int carry_of_plan(const Plan &p, uint32_t *out, unsigned long long CL, hipStream
   if (CL == 0) return 0;
   const unsigned long long wpb = (unsigned long long)CT * CWPT;
   unsigned long long nblk = (CL + wpb - 1) / wpb;
-  if (int rc = ensure(g.blk, nblk)) return rc;
+  if (int rc = ensure(g.blk, (nblk + 15) & ~15ull)) return rc;
   CarryArgs c;
   memset(&c, 0, sizeof c);
   c.ncoef = 2 * p.N;
@@ -552,9 +553,11 @@ int carry_of_plan(const Plan &p, uint32_t *out, unsigned long long CL, hipStream
   c.gC = p.C[0];
   c.gK = K;
   c.gKs = group_stride(K);
+  // LDS: the groups under one block's CT * CWPT words
+  const size_t glds = ((size_t)(32 * CT * CWPT) / (size_t)(p.C[0] * p.bits) + CARRY_GCOV + 2) * c.gKs * 4;
   {
     ProfScope ps(BCG_K_CARRY, s);
-    hipLaunchKernelGGL(k_carry_local, dim3((unsigned)nblk), dim3(CT), 0, s, c);
+    hipLaunchKernelGGL(k_carry_local, dim3((unsigned)nblk), dim3(CT), glds, s, c);
     LAUNCH(k_carry_fix, 1, 1024, s, c);
   }
   HIPCHK(hipGetLastError());
@@ -586,7 +589,7 @@ int read_margin(hipStream_t s, double *m) {
 // the plan keeps plain coefficients; group_stride rounds K up to 4 words.
 int group_stride(int K) { return (K + 3) & ~3; }
 int group_words(const Plan &p) {
-  if (p.P < 2 || (p.C[0] != 8 && p.C[0] != 16)) return 0;
+  if (p.P < 2 || (p.C[0] != 8 && p.C[0] != 16) || coef_bits(p) > 51) return 0;
   const int G = p.C[0], b = p.bits, cb = coef_bits(p);
   int off_max = 0;
   for (int g = 0; g < 32; g++) {
@@ -825,6 +828,8 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
   memset(&a, 0, sizeof a);
   a.desc = ddesc;
   a.margin = dmargins;
+  int fast = 1;  // every product of the class has coefficients below 2^51
+  for (const MulDesc &d : hdesc) fast = fast && d.cbits <= 51;
   for (int op = 0; op < 2; op++) {
     for (int i = 0; i < p.P - 1; i++) {
       PassArgs b = a;
@@ -855,6 +860,7 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
     if (int rc = pass_args(p, i, b)) return rc;
     b.data = da;
     b.coef = (long long *)db;
+    b.fast_round = fast;
     fwd_fn f = pick_inv(p.L[i], p.C[i]);
     if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
     ProfScope ps(i == 0 ? BCG_K_INV_UNPACK : BCG_K_INV, s);
@@ -868,6 +874,7 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
     unsigned long long nb = (d.cl + wpb - 1) / wpb;
     if (nb > nblk_max) nblk_max = nb;
   }
+  nblk_max = (nblk_max + 15) & ~15ull;  // 16-byte aligned rows of block functions
   if (int rc = ensure(g.bblk, nblk_max * (size_t)count)) return rc;
   CarryArgs c;
   memset(&c, 0, sizeof c);

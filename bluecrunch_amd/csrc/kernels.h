@@ -35,6 +35,7 @@ struct PassArgs {
   unsigned int *margin;     // float bits of the worst |x - round(x)| (atomic max)
   int margin_mask;          // single products: slot (block & mask) * MARGIN_STRIDE; batched: 0
   double inv_n;
+  int fast_round;           // every coefficient < 2^51: round with the 1.5 * 2^52 addition
   const MulDesc *desc;      // batched launch: product blockIdx.y, else nullptr
   int operand;              // batched first pass: 0 reads desc.a, 1 reads desc.b
   int coef_T;               // last pass: coefficient j at coef[(j % M) * 2L + j / M]
@@ -97,87 +98,106 @@ __device__ __forceinline__ void margin_update(unsigned int *m, double d) {
   atomicMax(m, __float_as_uint(f));
 }
 
+// Nearest integer of x clamped at 0, and the rounding distance into worst.
+// Equals fft/convert.c's (uint64_t)(x + 0.5) whenever |x - round x| < 1/2
+// (the multiply re-runs with smaller digits from 1/4 on). fast: |x| < 2^51,
+// where adding 1.5 * 2^52 leaves round-to-nearest(x) in the low mantissa bits.
+__device__ __forceinline__ long long round_coef(double x, bool fast, double &worst) {
+  if (fast) {
+    const double magic = 6755399441055744.0;  // 1.5 * 2^52
+    const double y = x + magic;
+    worst = fmax(worst, fabs(x - (y - magic)));
+    const long long v = __double_as_longlong(y) - __double_as_longlong(magic);
+    return v > 0 ? v : 0;
+  }
+  worst = fmax(worst, fabs(x - rint(x)));
+  const long long v = (long long)(x + 0.5);
+  return v > 0 ? v : 0;
+}
+
 // Round one output point j = row * M + col, already multiplied by
 // Omega^{-j} (f = w_j Omega^{-j}, unscaled), into coefficients c_j (real part)
 // and c_{j+N} (imaginary part); returns the rounding margin. L is the pass
 // length of the transposed layout (coef_T only).
 __device__ __forceinline__ double round_store(const PassArgs &a, cd f, unsigned long long row,
                                               unsigned long long col, unsigned long long L) {
-  double re = f.x * a.inv_n, im = f.y * a.inv_n;
-  double rr = rint(re), ri = rint(im);
-  double m = fmax(fabs(re - rr), fabs(im - ri));
-  // trunc(x + 0.5) like fft/convert.c's (uint64_t)(f + 0.5), clamped at 0
-  long long cr = (long long)(re + 0.5), ci = (long long)(im + 0.5);
+  double m = 0.0;
+  const long long cr = round_coef(f.x * a.inv_n, a.fast_round, m);
+  const long long ci = round_coef(f.y * a.inv_n, a.fast_round, m);
   if (a.coef_T) {
     // column-major [col][2L] (real part rows, then imaginary)
-    a.coef[col * 2 * L + row] = cr > 0 ? cr : 0;
-    a.coef[col * 2 * L + L + row] = ci > 0 ? ci : 0;
+    a.coef[col * 2 * L + row] = cr;
+    a.coef[col * 2 * L + L + row] = ci;
   } else {
     const unsigned long long j = row * a.M + col;
-    a.coef[j] = cr > 0 ? cr : 0;
-    a.coef[j + a.N] = ci > 0 ? ci : 0;
+    a.coef[j] = cr;
+    a.coef[j + a.N] = ci;
   }
   return m;
 }
 
-// Rounded coefficient pair of f = w_j Omega^{-j} (unscaled); updates worst.
-__device__ __forceinline__ void round_pair(const PassArgs &a, cd f, long long &cr, long long &ci, double &worst) {
-  double re = f.x * a.inv_n, im = f.y * a.inv_n;
-  worst = fmax(worst, fmax(fabs(re - rint(re)), fabs(im - rint(im))));
-  cr = (long long)(re + 0.5);
-  ci = (long long)(im + 0.5);
-  cr = cr > 0 ? cr : 0;
-  ci = ci > 0 ? ci : 0;
+// Exact sum of C coefficients v[i] 2^{b i + pos0} (pos0 < 32) as K words
+// (K >= the words the sum spans) into out[0..K) (LDS or global).
+template <int C>
+__device__ __forceinline__ void group_sum_words(const unsigned long long (&v)[C], int pos0, int b, int K,
+                                                uint32_t *out) {
+  // 128-bit accumulator (hi:lo); bit 0 is bit 32 k of the sum
+  unsigned long long lo = 0, hi = 0;
+  int pos = pos0, k = 0;
+#pragma unroll
+  for (int i = 0; i < C; i++) {
+    if (pos >= 32) {
+      out[k++] = (uint32_t)lo;
+      lo = (lo >> 32) | (hi << 32);
+      hi >>= 32;
+      pos -= 32;
+    }
+    const unsigned long long add_lo = v[i] << pos;
+    const unsigned long long add_hi = pos ? (v[i] >> (64 - pos)) : 0ull;
+    lo += add_lo;
+    hi += add_hi + (lo < add_lo ? 1ull : 0ull);
+    pos += b;
+  }
+  for (; k < K; k++) {
+    out[k] = (uint32_t)lo;
+    lo = (lo >> 32) | (hi << 32);
+    hi >>= 32;
+  }
 }
 
-// Group sums of the last inverse pass (group mode), one half (h = 0: real
-// parts, digits j < N; h = 1: imaginary parts, j >= N) at a time: row t of
-// the tile holds the C coefficients of digits j0 + [0, C), j0 = t M + m0 + h N,
-// at cl[t (C + 1) ...] (one pad slot per row against LDS bank conflicts).
-// Their sum sum_i c_i 2^{b i} is written exactly, shifted by (j0 b mod 32),
-// as ygK little-endian words (ygK a multiple of 4, zero padded) at
-// ybuf[(j0 / C) ygK ...]: the words from word floor(j0 b / 32) up. Words are
-// emitted into the row's own LDS slots (already consumed) and leave as
-// 16-byte stores.
+// Group sums of the last inverse pass (group mode). After the final stage the
+// tile slot (row t, column c) holds the rounded coefficient pair (c_j, c_{j+N})
+// of j = t M + m0 + c. For each row, the C real parts (digits j0 + [0, C),
+// j0 = t M + m0) and the C imaginary parts (j0 + N + [0, C)) each give one
+// exact sum sum_i c_i 2^{b i}, written shifted by (j0 b mod 32) as ygK
+// little-endian words (ygK a multiple of 4, zero padded) at ybuf[(j0 / C)
+// ygK ...]: the words from word floor(j0 b / 32) up. One thread per row; the
+// words are assembled in the row's own LDS slots and leave as 16-byte stores.
 template <int L, int C>
-__device__ __forceinline__ void write_group_sums(const PassArgs &a, long long *cl, unsigned long long m0, int half,
-                                                 int tid) {
+__device__ __forceinline__ void write_group_sums(const PassArgs &a, cd *lds, unsigned long long m0, int tid) {
   using TC = Tile<L, C>;
   const int b = a.bits;
   const int K = a.ygK;
   for (int t = tid; t < L; t += TC::T) {
-    const unsigned long long j0 = (unsigned long long)t * a.M + m0 + (half ? a.N : 0ull);
-    const long long *src = cl + t * (C + 1);
-    uint32_t *lw = reinterpret_cast<uint32_t *>(cl + t * (C + 1));
-    // 128-bit accumulator (hi:lo); bit 0 is bit 32 k of the group sum.
-    unsigned long long lo = 0, hi = 0;
-    int pos = (int)((j0 * (unsigned long long)b) & 31ull);
-    int k = 0;
-    unsigned long long cv[C];  // all loads before the first (aliasing) word store
-#pragma unroll
-    for (int i = 0; i < C; i++) cv[i] = (unsigned long long)src[i];
+    const int s0 = pad16(t * C);  // the row's C slots are contiguous
+    unsigned long long re[C], im[C];
 #pragma unroll
     for (int i = 0; i < C; i++) {
-      const unsigned long long v = cv[i];
-      if (pos >= 32) {
-        lw[k++] = (uint32_t)lo;
-        lo = (lo >> 32) | (hi << 32);
-        hi >>= 32;
-        pos -= 32;
-      }
-      const unsigned long long add_lo = v << pos;
-      const unsigned long long add_hi = pos ? (v >> (64 - pos)) : 0ull;
-      lo += add_lo;
-      hi += add_hi + (lo < add_lo ? 1ull : 0ull);
-      pos += b;
+      const longlong2 v = reinterpret_cast<const longlong2 *>(lds)[s0 + i];
+      re[i] = (unsigned long long)v.x;
+      im[i] = (unsigned long long)v.y;
     }
-    for (; k < K; k++) {
-      lw[k] = (uint32_t)lo;
-      lo = (lo >> 32) | (hi << 32);
-      hi >>= 32;
+    uint32_t *lw = reinterpret_cast<uint32_t *>(lds + s0);
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      const unsigned long long j0 = (unsigned long long)t * a.M + m0 + (half ? a.N : 0ull);
+      const int pos0 = (int)((j0 * (unsigned long long)b) & 31ull);
+      group_sum_words<C>(half ? im : re, pos0, b, K, lw + half * K);
+      uint4 *dst = reinterpret_cast<uint4 *>(a.ybuf + (j0 / C) * (unsigned long long)K);
+      for (int q = 0; q < K / 4; q++)
+        dst[q] = make_uint4(lw[half * K + 4 * q], lw[half * K + 4 * q + 1], lw[half * K + 4 * q + 2],
+                            lw[half * K + 4 * q + 3]);
     }
-    uint4 *dst = reinterpret_cast<uint4 *>(a.ybuf + (j0 / C) * (unsigned long long)K);
-    for (int q = 0; q < K / 4; q++) dst[q] = make_uint4(lw[4 * q], lw[4 * q + 1], lw[4 * q + 2], lw[4 * q + 3]);
   }
 }
 
@@ -357,15 +377,17 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
       lds[pad16(e)] = v;
     }
   } else {
-    // conjugate pass twiddle, rows r0 + h * L/16 as in k_fwd
+    // all sixteen loads in flight first, then the conjugate pass twiddle
+    // (rows r0 + h * L/16 as in k_fwd) on the way into LDS
+    cd xs[16];
+#pragma unroll
+    for (int h = 0; h < 16; h++) xs[h] = base[(unsigned long long)(r0 + h * (L / 16)) * a.M + m];
     Walk16 w;
     if (a.M > 1) w.template init_shared<C>(a.roots, a.tw_mul * m * 16ull * brev((unsigned)r0, TC::LOG - 4), a.tw_mul * m);
 #pragma unroll
     for (int h = 0; h < 16; h++) {
       const int row = r0 + h * (L / 16);
-      cd v = base[(unsigned long long)row * a.M + m];
-      if (a.M > 1) v = cmulc(v, w.at(h));
-      lds[pad16(row * C + c)] = v;
+      lds[pad16(row * C + c)] = (a.M > 1) ? cmulc(xs[h], w.at(h)) : xs[h];
     }
   }
   __syncthreads();
@@ -378,7 +400,6 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
   cd wun = make_double2(1.0, 0.0);
   if (UNPACK) wun = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
   const bool groups = UNPACK && a.ybuf;
-  long long cre[UNPACK ? 16 : 1], cim[UNPACK ? 16 : 1];
 #pragma unroll
   for (int i = 0; i < PER0; i++) {
     int q = r0 + i * (TC::T / C);
@@ -392,10 +413,15 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
       if constexpr (UNPACK) {
         const int h = i + t * PER0;
         const cd f = cmulc(x[t], cmul(wun, w64(h)));
-        if (groups)
-          round_pair(a, f, cre[h], cim[h], worst);
-        else
+        if (groups) {
+          // in place: this thread alone reads and writes these slots in the final stage
+          // group mode implies coefficients below 2^51
+          const long long cr = round_coef(f.x * a.inv_n, true, worst);
+          const long long ci = round_coef(f.y * a.inv_n, true, worst);
+          reinterpret_cast<longlong2 *>(lds)[pad16((q + t * SUB0) * C + c)] = make_longlong2(cr, ci);
+        } else {
           worst = fmax(worst, round_store(a, f, row, m, L));
+        }
       } else {
         base[row * a.M + m] = x[t];
       }
@@ -403,20 +429,8 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
   }
   if constexpr (UNPACK) {
     if (groups) {
-      // coefficients through LDS (row segments of C), then one exact group
-      // sum per segment
-      long long *cl = reinterpret_cast<long long *>(lds);
-#pragma unroll
-      for (int half = 0; half < 2; half++) {
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < 16; h++) {
-          const int row = r0 + h * (L / 16);
-          cl[row * (C + 1) + c] = half ? cim[h] : cre[h];
-        }
-        __syncthreads();
-        write_group_sums<L, C>(a, cl, m0, half, tid);
-      }
+      __syncthreads();
+      write_group_sums<L, C>(a, lds, m0, tid);
     }
     margin_reduce(a.margin, worst, a.margin_mask);
   }
@@ -718,9 +732,10 @@ __device__ __forceinline__ long long div_small(long long x, int b, double inv_b)
 
 // P_w from group sums (see write_group_sums): group g (digits [G g, G g + G))
 // covers words W_g .. W_g + K - 1, W_g = floor(G b g / 32); at most
-// CARRY_GCOV groups cover one word.
+// CARRY_GCOV groups cover one word. gl holds groups g_first... (stride gKs).
 constexpr int CARRY_GCOV = 4;
-__device__ __forceinline__ unsigned long long partial_from_groups(const CarryArgs &a, long long w, double inv_gb) {
+__device__ __forceinline__ unsigned long long partial_from_groups(const CarryArgs &a, const uint32_t *gl,
+                                                                  long long g_first, long long w, double inv_gb) {
   if (w < 0) return 0ull;
   const long long gb = (long long)a.gC * a.bits;
   long long g = div_small(32 * w + 31, (int)gb, inv_gb);
@@ -731,7 +746,7 @@ __device__ __forceinline__ unsigned long long partial_from_groups(const CarryArg
   for (int t = 0; t < CARRY_GCOV; t++) {
     const long long gg = g - t;
     const long long wg = (gg * gb) >> 5;
-    if (gg >= 0 && wg + a.gK > w && wg <= w) P += a.ybuf[gg * a.gKs + (w - wg)];
+    if (gg >= g_first && wg + a.gK > w && wg <= w) P += gl[(gg - g_first) * a.gKs + (w - wg)];
   }
   return P;
 }
@@ -777,6 +792,35 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
   if (jend >= (long long)a.ncoef) jend = (long long)a.ncoef - 1;
   const long long cnt = a.ybuf ? 0 : (jend >= jbase ? jend - jbase + 1 : 0);
   const bool staged = !a.ybuf && cnt <= (long long)a.lds_slots;
+  // group mode: the group sums under this block's words, staged with 16-byte
+  // loads (eight in flight per thread)
+  long long g_first = 0;
+  uint32_t *gl = reinterpret_cast<uint32_t *>(cl);
+  if (a.ybuf) {
+    const long long gb = (long long)a.gC * a.bits;
+    const double inv_gb = 1.0 / (double)gb;
+    const long long ng = (long long)(a.ncoef / (unsigned long long)a.gC);
+    g_first = W0 == 0 ? 0 : div_small(32 * (W0 - 1) + 31, (int)gb, inv_gb) - (CARRY_GCOV - 1);
+    if (g_first < 0) g_first = 0;
+    long long g_last = div_small(32 * (W0 + (long long)CT * CWPT - 1) + 31, (int)gb, inv_gb);
+    if (g_last >= ng) g_last = ng - 1;
+    const long long nvec = g_last >= g_first ? (g_last - g_first + 1) * a.gKs / 4 : 0;
+    const uint4 *src = reinterpret_cast<const uint4 *>(a.ybuf + g_first * a.gKs);
+    uint4 *dst = reinterpret_cast<uint4 *>(gl);
+    for (long long t0 = 0; t0 < nvec; t0 += 8 * CT) {
+      uint4 r[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const long long t = t0 + u * CT + tid;
+        r[u] = t < nvec ? src[t] : make_uint4(0u, 0u, 0u, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const long long t = t0 + u * CT + tid;
+        if (t < nvec) dst[t] = r[u];
+      }
+    }
+  }
   if (staged) {
     for (long long t0 = 0; t0 < cnt; t0 += 8 * CT) {
       long long r[8];
@@ -799,7 +843,7 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
 #pragma unroll
     for (int k = 0; k <= CWPT; k++) {
       long long w = wfirst - 1 + k;
-      P[k] = (w < (long long)a.CL) ? partial_from_groups(a, w, inv_gb) : 0ull;
+      P[k] = (w < (long long)a.CL) ? partial_from_groups(a, gl, g_first, w, inv_gb) : 0ull;
       if (a.add && w >= 0 && w < (long long)a.addl && w < (long long)a.CL) P[k] += a.add[w];
     }
   } else if (staged) {
@@ -875,48 +919,73 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
   }
 }
 
-// One workgroup: scan the block carry functions, then add the carry into every
-// block whose carry-in is 1 (ripple through 0xffffffff words inside the block).
+// One workgroup per product: scan the block carry functions, then add the
+// carry into every block whose carry-in is 1 (ripple through 0xffffffff words
+// inside the block). Thread t owns blocks [t per, t per + per) with per a
+// multiple of 16, so its function bytes arrive as 16-byte loads; the scan
+// across threads uses wave shuffles; the first words of the carried blocks
+// of a 16-block chunk are read together.
 static __global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
-  __shared__ unsigned char fns[1024];
-  const int tid = threadIdx.x;
+  __shared__ unsigned wfn[16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   bind_carry(a, blockIdx.x);
-  const unsigned long long per = (a.nblk + 1023) / 1024;
-  const unsigned long long b0 = tid * per;
+  const unsigned long long nb = a.nblk;
+  const unsigned long long per = (((nb + 1023) / 1024) + 15) & ~15ull;
+  const unsigned long long b0 = (unsigned long long)tid * per;
   unsigned long long b1 = b0 + per;
-  if (b1 > a.nblk) b1 = a.nblk;
-  unsigned g0 = 0, g1 = 1;
-  for (unsigned long long b = b0; b < b1; b++) {
-    unsigned f = a.blk_fn[b];
-    unsigned n0 = (g0 ? (f >> 1) : f) & 1u;
-    unsigned n1 = (g1 ? (f >> 1) : f) & 1u;
-    g0 = n0;
-    g1 = n1;
+  if (b1 > nb) b1 = nb;
+  unsigned fn = 2u;  // identity
+  for (unsigned long long c0 = b0; c0 < b1; c0 += 16) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(a.blk_fn + c0);
+    const unsigned wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+      if (c0 + k < b1) fn = carry_compose(fn, (wv[k >> 2] >> (8 * (k & 3))) & 3u);
   }
-  fns[tid] = (unsigned char)(g0 | (g1 << 1));
-  __syncthreads();
-  __shared__ unsigned char cin_of[1024];
-  if (tid == 0) {
-    unsigned c = 0;
-    for (int t = 0; t < 1024; t++) {
-      cin_of[t] = (unsigned char)c;
-      unsigned f = fns[t];
-      c = (c ? (f >> 1) : f) & 1u;
-    }
+  unsigned inc = fn;
+  for (int off = 1; off < 64; off <<= 1) {
+    unsigned gq = __shfl_up(inc, off, 64);
+    if (lane >= off) inc = carry_compose(gq, inc);
   }
+  unsigned excl = __shfl_up(inc, 1, 64);
+  if (lane == 0) excl = 2u;
+  if (lane == 63) wfn[wave] = inc;
   __syncthreads();
-  unsigned c = cin_of[tid];
+  unsigned pre = 2u;
+  for (int w = 0; w < wave; w++) pre = carry_compose(pre, wfn[w]);
+  unsigned c = carry_compose(pre, excl) & 1u;
   const unsigned long long wpb = (unsigned long long)CT * CWPT;
-  for (unsigned long long b = b0; b < b1; b++) {
-    unsigned f = a.blk_fn[b];
-    if (c) {
-      for (unsigned long long w = b * wpb; w < (b + 1) * wpb && w < a.CL; w++) {
-        uint32_t x = a.out[w] + 1u;
-        a.out[w] = x;
-        if (x != 0u) break;
+  for (unsigned long long c0 = b0; c0 < b1; c0 += 16) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(a.blk_fn + c0);
+    const unsigned wv[4] = {v.x, v.y, v.z, v.w};
+    unsigned carried = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      if (c0 + k < b1) {
+        const unsigned f = (wv[k >> 2] >> (8 * (k & 3))) & 3u;
+        if (c) carried |= 1u << k;
+        c = (f >> c) & 1u;
       }
     }
-    c = (c ? (f >> 1) : f) & 1u;
+    uint32_t x[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const unsigned long long w = (c0 + k) * wpb;
+      x[k] = ((carried >> k) & 1u) && w < a.CL ? a.out[w] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      unsigned long long w = (c0 + k) * wpb;
+      if (!((carried >> k) & 1u) || w >= a.CL) continue;
+      uint32_t y = x[k] + 1u;
+      a.out[w] = y;
+      // rare: the +1 runs through 0xffffffff words
+      const unsigned long long wend = (c0 + k + 1) * wpb;
+      while (y == 0u && ++w < wend && w < a.CL) {
+        y = a.out[w] + 1u;
+        a.out[w] = y;
+      }
+    }
   }
 }
 
@@ -1075,11 +1144,9 @@ __global__ __launch_bounds__((Tile<L, 1>::T)) void k_mul1(ClassArgs ca) {
       for (int t = 0; t < R0; t++) {
         int j = q + t * SUB0;
         cd f = cmulc(xa[i * R0 + t], omega_pow(a.roots, (unsigned long long)j));
-        double re = f.x * a.inv_n, im = f.y * a.inv_n;
-        worst = fmax(worst, fmax(fabs(re - rint(re)), fabs(im - rint(im))));
-        long long cr = (long long)(re + 0.5), ci = (long long)(im + 0.5);
-        coef[j] = cr > 0 ? cr : 0;
-        coef[j + L] = ci > 0 ? ci : 0;
+        const bool fast = d.cbits <= 51;
+        coef[j] = round_coef(f.x * a.inv_n, fast, worst);
+        coef[j + L] = round_coef(f.y * a.inv_n, fast, worst);
       }
     }
     margin_reduce(ca.margins + idx, worst);

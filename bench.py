@@ -29,21 +29,33 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 
 def algorithmic_bytes(kind, info, al, bl, cl):
-    """Compulsory HBM bytes of one launch of each kernel kind (16 B per complex point)."""
+    """Compulsory HBM bytes of one launch of each kernel kind (16 B per complex point).
+
+    The forward passes of A and B share one launch (grid.y = 2) unless squaring.
+    The last inverse pass writes either 2N 64-bit coefficients or, in group
+    mode, one exact sum of G digits per group (group_stride words each).
+    """
     n = info["points"]
     sq = info["square"]
+    ops = 1 if sq else 2
+    if info.get("group_digits"):
+        carry_in = (2 * n // info["group_digits"]) * info["group_stride"] * 4
+    else:
+        carry_in = 16 * n
     if kind == "fwd_pack":
-        return 4 * al + 16 * n  # read one operand's words, write its first-pass array
-    if kind in ("fwd", "inv"):
-        return 32 * n  # read and write one array
+        return 4 * (al if sq else al + bl) + 16 * n * ops  # read the words, write the first-pass arrays
+    if kind == "fwd":
+        return 32 * n * ops  # read and write each array
+    if kind == "inv":
+        return 32 * n
     if kind == "mid":
         if info["passes"] == 1:
-            return 4 * (al + (0 if sq else bl)) + 16 * n
+            return 4 * (al + (0 if sq else bl)) + 4 * cl
         return (32 if sq else 48) * n  # read A (and B) arrays, write the product array
     if kind == "inv_unpack":
-        return 32 * n  # read the product array, write 2N 8-byte coefficients
+        return 16 * n + carry_in  # read the product array, write coefficients / group sums
     if kind == "carry":
-        return 16 * n + 4 * cl  # read coefficients, write words
+        return carry_in + 4 * cl  # read coefficients / group sums, write words
     return 0
 
 

@@ -53,7 +53,8 @@ PROF_KINDS = {
 
 class MulInfo(ctypes.Structure):
     _fields_ = [("bits", ctypes.c_int), ("passes", ctypes.c_int), ("points", _sz), ("margin", ctypes.c_double),
-                ("retries", ctypes.c_int), ("square", ctypes.c_int)]
+                ("retries", ctypes.c_int), ("square", ctypes.c_int), ("group_digits", ctypes.c_int),
+                ("group_stride", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -513,6 +513,7 @@ int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbi
   c.coef_T = cM ? 1 : 0;
   c.cM = cM;
   c.crows = crows;
+  c.wpb = (unsigned)wpb;
   c.lds_slots = carry_slots(bits, cbits);
   {
     ProfScope ps(BCG_K_CARRY, s);
@@ -535,11 +536,12 @@ int carry_of_plan(const Plan &p, uint32_t *out, unsigned long long CL, hipStream
     return run_carry((const long long *)g.bufB.p, 2 * p.N, p.bits, coef_bits(p), out, CL, s, add, addl,
                      coef_transposed(p) ? (unsigned)p.M[0] : 0u, 2u * (unsigned)p.L[0]);
   if (CL == 0) return 0;
-  const unsigned long long wpb = (unsigned long long)CT * CWPT;
+  const unsigned long long wpb = (unsigned long long)GCT * GWPT;
   unsigned long long nblk = (CL + wpb - 1) / wpb;
   if (int rc = ensure(g.blk, (nblk + 15) & ~15ull)) return rc;
   CarryArgs c;
   memset(&c, 0, sizeof c);
+  c.wpb = (unsigned)wpb;
   c.ncoef = 2 * p.N;
   c.bits = p.bits;
   c.cbits = coef_bits(p);
@@ -553,11 +555,11 @@ int carry_of_plan(const Plan &p, uint32_t *out, unsigned long long CL, hipStream
   c.gC = p.C[0];
   c.gK = K;
   c.gKs = group_stride(K);
-  // LDS: the groups under one block's CT * CWPT words
-  const size_t glds = ((size_t)(32 * CT * CWPT) / (size_t)(p.C[0] * p.bits) + CARRY_GCOV + 2) * c.gKs * 4;
+  // LDS: the groups under one block's GCT * GWPT words
+  const size_t glds = ((size_t)(32 * GCT * GWPT) / (size_t)(p.C[0] * p.bits) + CARRY_GCOV + 2) * c.gKs * 4;
   {
     ProfScope ps(BCG_K_CARRY, s);
-    hipLaunchKernelGGL(k_carry_local, dim3((unsigned)nblk), dim3(CT), glds, s, c);
+    hipLaunchKernelGGL(k_carry_groups, dim3((unsigned)nblk), dim3(GCT), glds, s, c);
     LAUNCH(k_carry_fix, 1, 1024, s, c);
   }
   HIPCHK(hipGetLastError());
@@ -638,20 +640,24 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     return 0;
   }
   // forward outer passes for A (and B)
-  for (int op = 0; op < (square ? 1 : 2); op++) {
-    cd *buf = op == 0 ? da : db;
-    for (int i = 0; i < p.P - 1; i++) {
-      PassArgs b = a;
-      if (int rc = pass_args(p, i, b)) return rc;
-      b.data = buf;
-      b.wa = op == 0 ? A : B;
-      b.al = op == 0 ? AL : BL;
-      fwd_fn f = pick_fwd(p.L[i], p.C[i]);
-      if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
-      ProfScope ps(i == 0 ? BCG_K_FWD_PACK : BCG_K_FWD, s);
-      if (f(i == 0, b.ntiles, 1, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
-      HIPCHK(hipGetLastError());
+  // (A and B share each launch: grid.y = 2)
+  for (int i = 0; i < p.P - 1; i++) {
+    PassArgs b = a;
+    if (int rc = pass_args(p, i, b)) return rc;
+    b.data = da;
+    b.wa = A;
+    b.al = AL;
+    if (!square) {
+      b.pair = 1;
+      b.data_b = db;
+      b.wb = B;
+      b.bl = BL;
     }
+    fwd_fn f = pick_fwd(p.L[i], p.C[i]);
+    if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
+    ProfScope ps(i == 0 ? BCG_K_FWD_PACK : BCG_K_FWD, s);
+    if (f(i == 0, b.ntiles, square ? 1 : 2, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
+    HIPCHK(hipGetLastError());
   }
   // middle: last forward pass + pointwise + first inverse pass, in place in A
   {
@@ -884,6 +890,7 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
   c.ncoef = 2 * p.N;
   c.blk_fn = (unsigned char *)g.bblk.p;
   c.nblk = nblk_max;
+  c.wpb = (unsigned)wpb;
   {
     int bmin = 64, cbmax = 0;
     for (const MulDesc &d : hdesc) {
@@ -967,6 +974,8 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
     inf.passes = p.P;
     inf.points = (size_t)p.N;
     inf.margin = m;
+    inf.group_digits = group_words(p) ? p.C[0] : 0;
+    inf.group_stride = group_words(p) ? group_stride(group_words(p)) : 0;
     if (m >= 0.25) {
       if (p.bits <= 4) {
         if (info) *info = inf;

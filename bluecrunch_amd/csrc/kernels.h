@@ -41,12 +41,20 @@ struct PassArgs {
   int coef_T;               // last pass: coefficient j at coef[(j % M) * 2L + j / M]
   uint32_t *ybuf;           // last pass, group mode: digits [C g, C g + C) summed into ybuf[g K .. g K + K)
   int ygK;                  // words per group sum (a multiple of 4: 16-byte stores)
+  int pair;                 // forward passes of A and B in one launch: grid.y = 1 runs B (data_b, wb, bl)
 };
 
 // Batched launches (desc != nullptr): product p = blockIdx.y owns the arrays
 // data + p*N, data_b + p*N, coef + p*2N and margin slot p.
 __device__ __forceinline__ void bind_product(PassArgs &a) {
-  if (!a.desc) return;
+  if (!a.desc) {
+    if (a.pair && blockIdx.y == 1) {
+      a.data = const_cast<cd *>(a.data_b);
+      a.wa = a.wb;
+      a.al = a.bl;
+    }
+    return;
+  }
   const unsigned long long p = blockIdx.y;
   const MulDesc &d = a.desc[p];
   a.wa = a.operand ? d.b : d.a;
@@ -654,6 +662,7 @@ struct CarryArgs {
   unsigned lds_slots;        // int64 slots of dynamic LDS per block (coefficient window)
   const uint32_t *ybuf;      // group mode: group sums instead of coefficients (ncoef digits in all)
   int gC, gK, gKs;           // digits per group, words per group sum, group stride (words)
+  unsigned wpb;              // words per carry block (k_carry_fix)
 };
 
 __device__ __forceinline__ long long coef_at(const CarryArgs &a, long long j) {
@@ -730,26 +739,7 @@ __device__ __forceinline__ long long div_small(long long x, int b, double inv_b)
   return q;
 }
 
-// P_w from group sums (see write_group_sums): group g (digits [G g, G g + G))
-// covers words W_g .. W_g + K - 1, W_g = floor(G b g / 32); at most
-// CARRY_GCOV groups cover one word. gl holds groups g_first... (stride gKs).
-constexpr int CARRY_GCOV = 4;
-__device__ __forceinline__ unsigned long long partial_from_groups(const CarryArgs &a, const uint32_t *gl,
-                                                                  long long g_first, long long w, double inv_gb) {
-  if (w < 0) return 0ull;
-  const long long gb = (long long)a.gC * a.bits;
-  long long g = div_small(32 * w + 31, (int)gb, inv_gb);
-  const long long ng = (long long)(a.ncoef / (unsigned long long)a.gC);
-  if (g >= ng) g = ng - 1;
-  unsigned long long P = 0;
-#pragma unroll
-  for (int t = 0; t < CARRY_GCOV; t++) {
-    const long long gg = g - t;
-    const long long wg = (gg * gb) >> 5;
-    if (gg >= g_first && wg + a.gK > w && wg <= w) P += gl[(gg - g_first) * a.gKs + (w - wg)];
-  }
-  return P;
-}
+constexpr int CARRY_GCOV = 4;  // most group sums covering one word
 
 // P_w of one word from the LDS window cl[] (coefficient jbase at cl[0]).
 __device__ __forceinline__ unsigned long long partial_from_lds(const long long *cl, long long jbase, long long ncoef,
@@ -790,37 +780,8 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
   long long jbase = lo_bit <= 0 ? 0 : div_small(lo_bit + a.bits - 1, a.bits, inv_b);
   long long jend = div_small(32 * (W0 + (long long)CT * CWPT) - 1, a.bits, inv_b);  // inclusive
   if (jend >= (long long)a.ncoef) jend = (long long)a.ncoef - 1;
-  const long long cnt = a.ybuf ? 0 : (jend >= jbase ? jend - jbase + 1 : 0);
-  const bool staged = !a.ybuf && cnt <= (long long)a.lds_slots;
-  // group mode: the group sums under this block's words, staged with 16-byte
-  // loads (eight in flight per thread)
-  long long g_first = 0;
-  uint32_t *gl = reinterpret_cast<uint32_t *>(cl);
-  if (a.ybuf) {
-    const long long gb = (long long)a.gC * a.bits;
-    const double inv_gb = 1.0 / (double)gb;
-    const long long ng = (long long)(a.ncoef / (unsigned long long)a.gC);
-    g_first = W0 == 0 ? 0 : div_small(32 * (W0 - 1) + 31, (int)gb, inv_gb) - (CARRY_GCOV - 1);
-    if (g_first < 0) g_first = 0;
-    long long g_last = div_small(32 * (W0 + (long long)CT * CWPT - 1) + 31, (int)gb, inv_gb);
-    if (g_last >= ng) g_last = ng - 1;
-    const long long nvec = g_last >= g_first ? (g_last - g_first + 1) * a.gKs / 4 : 0;
-    const uint4 *src = reinterpret_cast<const uint4 *>(a.ybuf + g_first * a.gKs);
-    uint4 *dst = reinterpret_cast<uint4 *>(gl);
-    for (long long t0 = 0; t0 < nvec; t0 += 8 * CT) {
-      uint4 r[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const long long t = t0 + u * CT + tid;
-        r[u] = t < nvec ? src[t] : make_uint4(0u, 0u, 0u, 0u);
-      }
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const long long t = t0 + u * CT + tid;
-        if (t < nvec) dst[t] = r[u];
-      }
-    }
-  }
+  const long long cnt = jend >= jbase ? jend - jbase + 1 : 0;
+  const bool staged = cnt <= (long long)a.lds_slots;
   if (staged) {
     for (long long t0 = 0; t0 < cnt; t0 += 8 * CT) {
       long long r[8];
@@ -838,15 +799,7 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
   }
   __syncthreads();
   unsigned long long P[CWPT + 1];
-  if (a.ybuf) {
-    const double inv_gb = 1.0 / (double)(a.gC * a.bits);
-#pragma unroll
-    for (int k = 0; k <= CWPT; k++) {
-      long long w = wfirst - 1 + k;
-      P[k] = (w < (long long)a.CL) ? partial_from_groups(a, gl, g_first, w, inv_gb) : 0ull;
-      if (a.add && w >= 0 && w < (long long)a.addl && w < (long long)a.CL) P[k] += a.add[w];
-    }
-  } else if (staged) {
+  if (staged) {
     const int lane = tid & 63;
 #pragma unroll
     for (int k = 1; k <= CWPT; k++) {
@@ -919,6 +872,122 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
   }
 }
 
+// Carry resolve from group sums (see write_group_sums): group g (digits
+// [G g, G g + G)) covers words W_g .. W_g + K - 1, W_g = floor(G b g / 32),
+// and at most CARRY_GCOV groups cover one word. A block owns GCT * GWPT
+// words; the group sums under them are staged in LDS with 16-byte loads;
+// each thread walks its GWPT + 1 words (the first is its left neighbour's
+// last) with a running group index, forms P_w, v_w = lo(P_w) + hi(P_{w-1})
+// and the carry-function scan of k_carry_local.
+constexpr int GCT = 256;
+constexpr int GWPT = 16;
+static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
+  extern __shared__ uint32_t gl[];
+  __shared__ unsigned wave_fn[GCT / 64];
+  const int tid = threadIdx.x;
+  const long long W0 = (long long)blockIdx.x * GCT * GWPT;
+  const int gb = a.gC * a.bits;  // bits per group, >= 32
+  const double inv_gb = 1.0 / (double)gb;
+  const long long ng = (long long)(a.ncoef / (unsigned long long)a.gC);
+  long long g_first = W0 == 0 ? 0 : div_small(32 * (W0 - 1) + 31, gb, inv_gb) - (CARRY_GCOV - 1);
+  if (g_first < 0) g_first = 0;
+  long long g_last = div_small(32 * (W0 + (long long)GCT * GWPT - 1) + 31, gb, inv_gb);
+  if (g_last >= ng) g_last = ng - 1;
+  {
+    const long long nvec = g_last >= g_first ? (g_last - g_first + 1) * a.gKs / 4 : 0;
+    const uint4 *src = reinterpret_cast<const uint4 *>(a.ybuf + g_first * a.gKs);
+    uint4 *dst = reinterpret_cast<uint4 *>(gl);
+    for (long long t0 = 0; t0 < nvec; t0 += 8 * GCT) {
+      uint4 r[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const long long t = t0 + u * GCT + tid;
+        r[u] = t < nvec ? src[t] : make_uint4(0u, 0u, 0u, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const long long t = t0 + u * GCT + tid;
+        if (t < nvec) dst[t] = r[u];
+      }
+    }
+  }
+  __syncthreads();
+  const long long wfirst = W0 + (long long)tid * GWPT;
+  // running group: g = last group with W_g <= w; Wt[t] = W_{g - t}
+  long long w = wfirst - 1;
+  long long g = w < 0 ? 0 : div_small(32 * w + 31, gb, inv_gb);
+  if (g >= ng) g = ng - 1;
+  long long Wt[CARRY_GCOV];
+#pragma unroll
+  for (int t = 0; t < CARRY_GCOV; t++) Wt[t] = ((g - t) * gb) >> 5;
+  unsigned long long P[GWPT + 1];
+#pragma unroll
+  for (int k = 0; k <= GWPT; k++) {
+    w = wfirst - 1 + k;
+    if (k > 0 && g + 1 < ng && (((g + 1) * gb) >> 5) <= w) {
+      g++;
+#pragma unroll
+      for (int t = CARRY_GCOV - 1; t > 0; t--) Wt[t] = Wt[t - 1];
+      Wt[0] = (g * gb) >> 5;
+    }
+    unsigned long long p = 0;
+    if (w >= 0 && w < (long long)a.CL) {
+#pragma unroll
+      for (int t = 0; t < CARRY_GCOV; t++) {
+        const long long off = w - Wt[t];
+        if (g - t >= g_first && off < a.gK) p += gl[(g - t - g_first) * a.gKs + off];
+      }
+      if (a.add && w < (long long)a.addl) p += a.add[w];
+    }
+    P[k] = p;
+  }
+  if (wfirst == 0) P[0] = 0;
+  unsigned long long v[GWPT];
+#pragma unroll
+  for (int k = 0; k < GWPT; k++) v[k] = (P[k + 1] & 0xffffffffull) + (P[k] >> 32);
+  unsigned f0 = 0, f1 = 1;
+#pragma unroll
+  for (int k = 0; k < GWPT; k++) {
+    f0 = (unsigned)((v[k] + f0) >> 32);
+    f1 = (unsigned)((v[k] + f1) >> 32);
+  }
+  const unsigned f = f0 | (f1 << 1);
+  const int lane = tid & 63, wave = tid >> 6;
+  unsigned inc = f;
+  for (int off = 1; off < 64; off <<= 1) {
+    unsigned gq = __shfl_up(inc, off, 64);
+    if (lane >= off) inc = carry_compose(gq, inc);
+  }
+  unsigned excl = __shfl_up(inc, 1, 64);
+  if (lane == 0) excl = 2u;
+  if (lane == 63) wave_fn[wave] = inc;
+  __syncthreads();
+  unsigned pre = 2u;
+  for (int q = 0; q < wave; q++) pre = carry_compose(pre, wave_fn[q]);
+  if (tid == 0) {
+    unsigned tot = 2u;
+    for (int q = 0; q < GCT / 64; q++) tot = carry_compose(tot, wave_fn[q]);
+    a.blk_fn[blockIdx.x] = (unsigned char)tot;
+  }
+  unsigned cin = carry_compose(pre, excl) & 1u;
+  uint32_t o[GWPT];
+#pragma unroll
+  for (int k = 0; k < GWPT; k++) {
+    unsigned long long sum = v[k] + cin;
+    o[k] = (uint32_t)sum;
+    cin = (unsigned)(sum >> 32);
+  }
+  if (wfirst + GWPT <= (long long)a.CL && ((reinterpret_cast<uintptr_t>(a.out + wfirst) & 15) == 0)) {
+    uint4 *dst = reinterpret_cast<uint4 *>(a.out + wfirst);
+#pragma unroll
+    for (int q = 0; q < GWPT / 4; q++) dst[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < GWPT; k++)
+      if (wfirst + k < (long long)a.CL) a.out[wfirst + k] = o[k];
+  }
+}
+
 // One workgroup per product: scan the block carry functions, then add the
 // carry into every block whose carry-in is 1 (ripple through 0xffffffff words
 // inside the block). Thread t owns blocks [t per, t per + per) with per a
@@ -954,7 +1023,7 @@ static __global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
   unsigned pre = 2u;
   for (int w = 0; w < wave; w++) pre = carry_compose(pre, wfn[w]);
   unsigned c = carry_compose(pre, excl) & 1u;
-  const unsigned long long wpb = (unsigned long long)CT * CWPT;
+  const unsigned long long wpb = a.wpb;
   for (unsigned long long c0 = b0; c0 < b1; c0 += 16) {
     const uint4 v = *reinterpret_cast<const uint4 *>(a.blk_fn + c0);
     const unsigned wv[4] = {v.x, v.y, v.z, v.w};

@@ -34,6 +34,8 @@ typedef struct {
   double margin;     /* worst |x - round(x)| over all 2N output coefficients */
   int retries;       /* re-runs with a smaller digit size because margin >= 0.25 */
   int square;        /* 1 when the squaring path ran (one forward transform) */
+  int group_digits;  /* carry input: digits per exact group sum (8 or 16), 0 = 64-bit coefficients */
+  int group_stride;  /* words stored per group sum (0 with coefficients) */
 } bcg_mul_info;
 
 /* Device set-up: select device, create the stream, warm the kernels.

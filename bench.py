@@ -29,15 +29,18 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 
 def algorithmic_bytes(kind, info, al, bl, cl):
-    """Compulsory HBM bytes of one launch of each kernel kind (16 B per complex point).
+    """Compulsory HBM bytes of each kernel kind in one product (16 B per complex point).
 
-    The forward passes of A and B share one launch (grid.y = 2) unless squaring.
-    The last inverse pass writes either 2N 64-bit coefficients or, in group
-    mode, one exact sum of G digits per group (group_stride words each).
+    The forward passes run once per operand (A and B, or A alone when squaring);
+    a three-pass plan has one middle forward and one middle inverse pass. The
+    last inverse pass writes either 2N 64-bit coefficients or, in group mode,
+    one exact sum of G digits per group (group_stride words each). Per-launch
+    figures divide these by the launches of that kind in one product.
     """
     n = info["points"]
     sq = info["square"]
     ops = 1 if sq else 2
+    mids = max(info["passes"] - 2, 0)
     if info.get("group_digits"):
         carry_in = (2 * n // info["group_digits"]) * info["group_stride"] * 4
     else:
@@ -45,9 +48,9 @@ def algorithmic_bytes(kind, info, al, bl, cl):
     if kind == "fwd_pack":
         return 4 * (al if sq else al + bl) + 16 * n * ops  # read the words, write the first-pass arrays
     if kind == "fwd":
-        return 32 * n * ops  # read and write each array
+        return 32 * n * ops * mids  # read and write each array
     if kind == "inv":
-        return 32 * n
+        return 32 * n * mids
     if kind == "mid":
         if info["passes"] == 1:
             return 4 * (al + (0 if sq else bl)) + 4 * cl
@@ -189,7 +192,7 @@ def main():
         n, ms = bc.prof_read(kind)
         if n:
             per = ms / n
-            byt = algorithmic_bytes(kind, info, al, bl, cl)
+            byt = algorithmic_bytes(kind, info, al, bl, cl) * args.steps / n  # per launch
             kern[kind] = {"launches": n, "avg_ms": per, "alg_bytes": byt, "GBps": byt / (per * 1e-3) / 1e9}
     bc.prof_reset()
 
@@ -215,8 +218,10 @@ def main():
         ach = d["alg_bytes"] / (d["avg_ms"] * 1e-3)
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": ach / HBM_PEAK, "traffic": read_traffic(dom)}
-    fft_bytes = sum(v["alg_bytes"] * v["launches"] for k, v in kern.items() if k != "carry")
-    fft_ms = sum(v["avg_ms"] * v["launches"] for k, v in kern.items() if k != "carry")
+    # FFT passes (everything but the carry) per product: their bytes over the
+    # product's wall time less the carry kernels (passes may overlap in time)
+    fft_bytes = sum(v["alg_bytes"] * v["launches"] for k, v in kern.items() if k != "carry") / args.steps
+    carry_ms = kern["carry"]["avg_ms"] * kern["carry"]["launches"] / args.steps if "carry" in kern else 0.0
 
     cpu = None
     if not args.no_cpu:
@@ -249,7 +254,7 @@ def main():
         },
         "square": {"ms_per_step": sec_sq * 1e3, "digits_per_s": digits / sec_sq, "worst_margin": s_worst,
                    "bits_per_digit": s_info["bits"], "points": s_info["points"]},
-        "fft_passes_hbm_frac": (fft_bytes / (fft_ms * 1e-3)) / HBM_PEAK if fft_ms else None,
+        "fft_passes_hbm_frac": (fft_bytes / (sec - carry_ms * 1e-3)) / HBM_PEAK if sec > carry_ms * 1e-3 else None,
         "kernels": kern,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -556,7 +556,8 @@ int carry_of_plan(const Plan &p, uint32_t *out, unsigned long long CL, hipStream
   c.gK = K;
   c.gKs = group_stride(K);
   // LDS: the groups under one block's GCT * GWPT words
-  const size_t glds = ((size_t)(32 * GCT * GWPT) / (size_t)(p.C[0] * p.bits) + CARRY_GCOV + 2) * c.gKs * 4;
+  size_t glds = ((size_t)(32 * GCT * GWPT) / (size_t)(p.C[0] * p.bits) + CARRY_GCOV + 2) * c.gKs;
+  glds = (glds + glds / 32 + 4) * 4;  // one pad word per 32
   {
     ProfScope ps(BCG_K_CARRY, s);
     hipLaunchKernelGGL(k_carry_groups, dim3((unsigned)nblk), dim3(GCT), glds, s, c);

@@ -893,10 +893,11 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
   if (g_first < 0) g_first = 0;
   long long g_last = div_small(32 * (W0 + (long long)GCT * GWPT - 1) + 31, gb, inv_gb);
   if (g_last >= ng) g_last = ng - 1;
+  // LDS word i lives at i + i / 32: threads read words 16 apart, and the pad
+  // keeps those reads on distinct banks
   {
     const long long nvec = g_last >= g_first ? (g_last - g_first + 1) * a.gKs / 4 : 0;
     const uint4 *src = reinterpret_cast<const uint4 *>(a.ybuf + g_first * a.gKs);
-    uint4 *dst = reinterpret_cast<uint4 *>(gl);
     for (long long t0 = 0; t0 < nvec; t0 += 8 * GCT) {
       uint4 r[8];
 #pragma unroll
@@ -907,7 +908,13 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const long long t = t0 + u * GCT + tid;
-        if (t < nvec) dst[t] = r[u];
+        if (t < nvec) {
+          const unsigned i = (unsigned)(4 * t), pi = i + (i >> 5);  // 4 words never straddle a pad
+          gl[pi] = r[u].x;
+          gl[pi + 1] = r[u].y;
+          gl[pi + 2] = r[u].z;
+          gl[pi + 3] = r[u].w;
+        }
       }
     }
   }
@@ -935,7 +942,10 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
 #pragma unroll
       for (int t = 0; t < CARRY_GCOV; t++) {
         const long long off = w - Wt[t];
-        if (g - t >= g_first && off < a.gK) p += gl[(g - t - g_first) * a.gKs + off];
+        if (g - t >= g_first && off < a.gK) {
+          const unsigned i = (unsigned)((g - t - g_first) * a.gKs + off);
+          p += gl[i + (i >> 5)];
+        }
       }
       if (a.add && w < (long long)a.addl) p += a.add[w];
     }

@@ -85,6 +85,33 @@ def plan_split(bc, n):
     return {"L": list(L), "C": list(C)}
 
 
+def e_pipeline(bc, digits, compare_digits):
+    """e = 2.71828... to `digits` decimals on the GPU (batched binary splitting,
+    Newton division, radix conversion), wall time around the library call, MD5
+    of the text against the reference's md5sums (tests/golden/vectors.json).
+    At compare_digits the binary splitting also runs one call per product."""
+    import hashlib
+
+    with open(os.path.join(ROOT, "tests", "golden", "vectors.json")) as f:
+        gold = json.load(f)["e_md5"]
+    bc.e_digits(1000)  # warm-up (kernels, tables)
+    t0 = time.perf_counter()
+    text, ms = bc.e_digits(digits, raw=True)
+    wall = time.perf_counter() - t0
+    md5 = hashlib.md5(text).hexdigest()
+    out = {"digits": digits, "wall_s": wall, "bsr_ms": ms["bsr_ms"], "div_ms": ms["div_ms"],
+           "radix_ms": ms["radix_ms"], "md5": md5,
+           "md5_ok": (md5 == gold[str(digits)]) if str(digits) in gold else None}
+    if compare_digits:
+        _, mb = bc.e_digits(compare_digits)
+        _, mp = bc.e_digits(compare_digits, per_multiply=True)
+        out["bsr_compare"] = {"digits": compare_digits, "batched_bsr_ms": mb["bsr_ms"],
+                              "per_multiply_bsr_ms": mp["bsr_ms"],
+                              "speedup": mp["bsr_ms"] / mb["bsr_ms"] if mb["bsr_ms"] > 0 else None,
+                              "batched_total_ms": mb["total_ms"], "per_multiply_total_ms": mp["total_ms"]}
+    return out
+
+
 def cpu_baseline(words, seed):
     """Oracle _fft_mul on host cores for a bounded sample (words x words)."""
     from oracle import refcpu
@@ -123,6 +150,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default pass split)")
     ap.add_argument("--no-square", action="store_true", help="skip the squaring runs (profiler runs)")
+    ap.add_argument("--e-digits", type=int, default=10 ** 8, help="digits of e for the e leg (0: skip)")
+    ap.add_argument("--e-compare-digits", type=int, default=10 ** 6,
+                    help="digits of e for the batched vs per-multiply binary-splitting comparison (0: skip)")
     ap.add_argument("--split", default=None,
                     help="force the pass split, e.g. 64,256,4096 (skips the autotuner); for profiler runs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -231,6 +261,8 @@ def main():
     fft_bytes = sum(v["alg_bytes"] * v["launches"] for k, v in kern.items() if k != "carry") / args.steps
     carry_ms = kern["carry"]["avg_ms"] * kern["carry"]["launches"] / args.steps if "carry" in kern else 0.0
 
+    e_leg = e_pipeline(bc, args.e_digits, args.e_compare_digits) if args.e_digits else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_words, args.seed)
@@ -267,6 +299,7 @@ def main():
         "kernels": kern,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "e": e_leg,
         "speedup_vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
         "parity_fingerprint_ok": ok,
     }

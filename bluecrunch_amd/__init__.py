@@ -31,7 +31,7 @@ EXPORTS = {
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
-        "bcg_plan_force", "bcg_plan_current", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_prof_enable", "bcg_prof_read",
+        "bcg_plan_force", "bcg_plan_current", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_prof_enable", "bcg_prof_read",
         "bcg_prof_reset",
     ],
     "fft.h": [
@@ -121,6 +121,7 @@ def lib():
     L.bcg_div_fraction_dev.argtypes = [_vp, _sz, _vp, _sz, _sz, ctypes.POINTER(_vp)]
     L.bcg_radix_dev.argtypes = [_vp, _sz, _sz, _vp]
     L.bcg_e_digits.argtypes = [_sz, ctypes.c_char_p, _sz, _f64p]
+    L.bcg_e_digits_ex.argtypes = [_sz, ctypes.c_char_p, _sz, _f64p, i]
     L.bcg_prof_enable.argtypes = [i]
     L.bcg_prof_read.argtypes = [i, ctypes.POINTER(ctypes.c_long), _f64p]
     # reference C API
@@ -490,9 +491,14 @@ def radix(fraction_words, groups):
     return dD.to_host()
 
 
-def e_digits(digits):
-    """("2." + digits decimal digits of e, timings dict) computed on the GPU."""
+def e_digits(digits, per_multiply=False, raw=False):
+    """("2." + digits decimal digits of e, timings dict) computed on the GPU.
+
+    per_multiply: binary-splitting products one call each instead of batched.
+    raw: return the bytes object (no decode) for very long outputs."""
     buf = ctypes.create_string_buffer(digits + 3)
     ms = (ctypes.c_double * 4)()
-    check(lib().bcg_e_digits(digits, buf, digits + 3, ms), "bcg_e_digits")
-    return buf.value.decode(), {"bsr_ms": ms[0], "div_ms": ms[1], "radix_ms": ms[2], "total_ms": ms[3]}
+    check(lib().bcg_e_digits_ex(digits, buf, digits + 3, ms, 1 if per_multiply else 0), "bcg_e_digits")
+    val = ctypes.string_at(buf, digits + 2)
+    times = {"bsr_ms": ms[0], "div_ms": ms[1], "radix_ms": ms[2], "total_ms": ms[3]}
+    return (val if raw else val.decode()), times

@@ -1862,6 +1862,13 @@ int bcg_e_digits(size_t digits, char *out, size_t cap, double *ms) {
   return e_digits(digits, out, cap, ms, g.stream);
 }
 
+int bcg_e_digits_ex(size_t digits, char *out, size_t cap, double *ms, int flags) {
+  ENTER();
+  if (!out) return fail(BCG_E_ARG, "null pointer");
+  if (flags & ~BCG_E_PER_MULTIPLY) return fail(BCG_E_ARG, "unknown flags");
+  return e_digits(digits, out, cap, ms, g.stream, (flags & BCG_E_PER_MULTIPLY) != 0);
+}
+
 int bcg_radix_dev(const uint32_t *F, size_t nf, size_t groups, uint32_t *digits) {
   ENTER();
   if (!F || !nf || !digits) return fail(BCG_E_ARG, "bad arguments");

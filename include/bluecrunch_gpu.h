@@ -152,6 +152,11 @@ int bcg_radix_dev(const uint32_t *F, size_t nf, size_t groups, uint32_t *digits)
  * ms (optional, 4 entries): binary splitting, division, radix conversion,
  * total wall milliseconds. */
 int bcg_e_digits(size_t digits, char *out, size_t cap, double *ms);
+/* Same, with flags: BCG_E_PER_MULTIPLY runs the binary-splitting products one
+ * library call per product (the plain per-multiply GPU path) instead of one
+ * batched call per tree level. */
+#define BCG_E_PER_MULTIPLY 1
+int bcg_e_digits_ex(size_t digits, char *out, size_t cap, double *ms, int flags);
 int bcg_e_bsr_dev(size_t terms, uint32_t **pq, size_t *pl, size_t *ql);
 
 /* Kernel timing with HIP events recorded on the launch stream. While enabled,

@@ -92,3 +92,21 @@ def test_e_digits_1e6_md5(bc, golden):
     import hashlib
     txt, ms = bc.e_digits(1000000)
     assert hashlib.md5(txt.encode()).hexdigest() == golden["e_md5"]["1000000"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D", [1000, 100000])
+def test_e_digits_per_multiply_path(bc, golden, D):
+    # the plain per-multiply binary splitting gives the same digits as the batched one
+    import hashlib
+    txt, ms = bc.e_digits(D, per_multiply=True)
+    assert hashlib.md5(txt.encode()).hexdigest() == golden["e_md5"][str(D)]
+
+
+@pytest.mark.gpu
+def test_e_digits_1e7_md5(bc, golden):
+    # GPU-side decimal formatting at 1e7 digits (raw bytes, reference md5sums)
+    import hashlib
+    txt, ms = bc.e_digits(10 ** 7, raw=True)
+    assert len(txt) == 10 ** 7 + 2 and txt[:2] == b"2."
+    assert hashlib.md5(txt).hexdigest() == golden["e_md5"]["10000000"]

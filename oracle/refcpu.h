@@ -64,6 +64,7 @@ void ref_bigfloat_mul(ref_bigfloat *target, const ref_bigfloat *a, const ref_big
 
 /* Worker threads used by the parallel regions (OpenMP). */
 int ref_num_threads(void);
+void ref_set_threads(int n); /* OpenMP threads of later calls (n > 0) */
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,8 @@ def lib():
         L.ref_bigfloat_mul.argtypes = [ctypes.POINTER(RefBigfloat)] * 3 + [sz]
         L.ref_bigfloat_mul.restype = None
         L.ref_num_threads.restype = ctypes.c_int
+        L.ref_set_threads.argtypes = [ctypes.c_int]
+        L.ref_set_threads.restype = None
         _lib = L
     return _lib
 
@@ -170,3 +172,8 @@ def bigfloat_mul(a, b, p=0):
 
 def num_threads():
     return lib().ref_num_threads()
+
+
+def set_threads(n):
+    """OpenMP threads used by later oracle calls."""
+    lib().ref_set_threads(int(n))

@@ -112,6 +112,70 @@ def e_pipeline(bc, digits, compare_digits):
     return out
 
 
+def batch_leg(bc, count, seed):
+    """The batched workload: count independent products with log-uniform operand
+    lengths in [16, 2^14] words (seed 7), device-resident, as one batched library
+    call (bcg_mul_batch_dev: products grouped by length class, one persistent
+    launch per class) against one bcg_mul_dev call (one kernel set) per product."""
+    import ctypes
+    from bluecrunch_amd.workload import batch_operands
+
+    pairs = batch_operands(seed, count)
+    al = np.array([len(x) for x, _ in pairs], dtype=np.uint64)
+    bl = np.array([len(y) for _, y in pairs], dtype=np.uint64)
+    cl = al + bl
+    off = lambda n: np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.uint64)  # noqa: E731
+    a_off, b_off, c_off = off(al), off(bl), off(cl)
+    dA = bc.DeviceWords.from_host(np.concatenate([x for x, _ in pairs]))
+    dB = bc.DeviceWords.from_host(np.concatenate([y for _, y in pairs]))
+    dC = bc.DeviceWords(int(cl.sum()))
+    dC2 = bc.DeviceWords(int(cl.sum()))
+    L = bc.lib()
+    P = ctypes.POINTER(ctypes.c_size_t)
+    arrs = [np.ascontiguousarray(v, np.uint64) for v in (c_off, cl, a_off, al, b_off, bl)]
+    ptrs = [v.ctypes.data_as(P) for v in arrs]
+    worst = ctypes.c_double()
+
+    def batched():
+        bc.check(L.bcg_mul_batch_dev(dC.ptr, dA.ptr, dB.ptr, *ptrs, count, None, ctypes.byref(worst)), "batch")
+
+    info = bc.MulInfo()
+
+    def per_call():
+        for i in range(count):
+            bc.check(L.bcg_mul_dev(dC2.ptr + 4 * int(c_off[i]), int(cl[i]), dA.ptr + 4 * int(a_off[i]), int(al[i]),
+                                   dB.ptr + 4 * int(b_off[i]), int(bl[i]), None, ctypes.byref(info)), "mul")
+
+    batched()
+    bc.sync()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        batched()
+    bc.sync()
+    tb = (time.perf_counter() - t0) / 3
+    per_call()
+    bc.sync()
+    t0 = time.perf_counter()
+    per_call()
+    bc.sync()
+    tp = time.perf_counter() - t0
+    same = bool(np.array_equal(dC.to_host(), dC2.to_host()))
+    # exact spot checks against Python integers
+    from bluecrunch_amd.workload import words_to_int, int_to_words
+    C = dC.to_host()
+    exact = True
+    for i in range(0, count, max(1, count // 16)):
+        x, y = pairs[i]
+        want = int_to_words(words_to_int(x) * words_to_int(y), int(cl[i]))
+        exact = exact and bool(np.array_equal(C[int(c_off[i]):int(c_off[i]) + int(cl[i])], want))
+    for d in (dA, dB, dC, dC2):
+        d.free()
+    return {"products": count, "seed": seed, "total_operand_words": int(al.sum() + bl.sum()),
+            "batched_ms": tb * 1e3, "per_call_ms": tp * 1e3, "speedup": tp / tb,
+            "products_per_s": count / tb, "worst_margin": worst.value,
+            "batched_equals_per_call": same, "exact_spot_checks": exact}
+
+
 def cpu_baseline(words, seed, one_core_words):
     """Oracle _fft_mul (the CPU restatement of the reference AVX path, OpenMP) on
     the host cores: one warm-up then the median of 3 runs of a words x words
@@ -172,6 +236,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default pass split)")
     ap.add_argument("--no-square", action="store_true", help="skip the squaring runs (profiler runs)")
     ap.add_argument("--e-digits", type=int, default=10 ** 8, help="digits of e for the e leg (0: skip)")
+    ap.add_argument("--batch", type=int, default=4096, help="products in the batched leg (0: skip)")
     ap.add_argument("--e-compare-digits", type=int, default=10 ** 6,
                     help="digits of e for the batched vs per-multiply binary-splitting comparison (0: skip)")
     ap.add_argument("--split", default=None,
@@ -292,6 +357,7 @@ def main():
         host_ms = (time.perf_counter() - t4) / 3 * 1e3
 
     e_leg = e_pipeline(bc, args.e_digits, args.e_compare_digits) if args.e_digits else None
+    b_leg = batch_leg(bc, args.batch, 7) if args.batch else None
 
     cpu = None
     if not args.no_cpu:
@@ -333,6 +399,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "e": e_leg,
+        "batch": b_leg,
         "speedup_vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
         "parity_fingerprint_ok": ok,
     }

@@ -69,6 +69,19 @@ def batch_lengths(seed, count, lo=16, hi=16384):
     return np.clip(lens, lo, hi).reshape(count, 2)
 
 
+def batch_operands(seed, count, lo=16, hi=16384):
+    """The batched workload: per product log-uniform lengths (al, bl) in
+    [lo, hi] as batch_lengths, then the words of A and B of each product in
+    turn from the same SplitMix64 stream (after all lengths)."""
+    rng = SplitMix64(seed)
+    u = rng.next_block(2 * count).astype(np.float64) / 2.0**64
+    lens = np.clip(np.floor(2.0 ** (4 + 10 * u)).astype(np.int64), lo, hi).reshape(count, 2)
+    pairs = []
+    for al, bl in lens:
+        pairs.append((operand(rng, int(al)), operand(rng, int(bl))))
+    return pairs
+
+
 def words_to_int(w):
     return int.from_bytes(np.ascontiguousarray(w, dtype="<u4").tobytes(), "little")
 

@@ -27,7 +27,7 @@ EXPORTS = {
     "bluecrunch_gpu.h": [
         "bcg_init", "bcg_shutdown", "bcg_last_error", "bcg_stream", "bcg_sync", "bcg_malloc", "bcg_free",
         "bcg_memcpy_h2d", "bcg_memcpy_d2h", "bcg_memcpy_d2d", "bcg_mul_dev", "bcg_mul_host", "bcg_mul_dev_bits", "bcg_basecase_dev",
-        "bcg_mul_batch_dev", "bcg_muladd_dev", "bcg_muladd_batch_dev", "bcg_fft_forward_dev", "bcg_fft_inverse_dev", "bcg_fft_pointwise_dev",
+        "bcg_mul_batch_dev", "bcg_muladd_dev", "bcg_muladd_batch_dev", "bcg_batch_create", "bcg_batch_run", "bcg_batch_destroy", "bcg_fft_forward_dev", "bcg_fft_inverse_dev", "bcg_fft_pointwise_dev",
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
@@ -98,6 +98,9 @@ def lib():
     L.bcg_mul_batch_dev.argtypes = [_vp, _vp, _vp] + [ctypes.POINTER(_sz)] * 6 + [_sz, _vp, _f64p]
     L.bcg_muladd_dev.argtypes = mul_args + [_vp, _sz, _vp, ctypes.POINTER(MulInfo)]
     L.bcg_muladd_batch_dev.argtypes = [_vp, _vp, _vp, _vp] + [ctypes.POINTER(_sz)] * 8 + [_sz, _vp, _f64p]
+    L.bcg_batch_create.argtypes = [ctypes.POINTER(_vp), _vp, _vp, _vp, _vp] + [ctypes.POINTER(_sz)] * 8 + [_sz]
+    L.bcg_batch_run.argtypes = [_vp, _vp, _f64p]
+    L.bcg_batch_destroy.argtypes = [_vp]
     L.bcg_fft_forward_dev.argtypes = [_vp, _sz, _vp]
     L.bcg_fft_inverse_dev.argtypes = [_vp, _sz, _vp]
     L.bcg_fft_pointwise_dev.argtypes = [_vp, _vp, _sz, _vp]
@@ -439,6 +442,40 @@ def mul_batch(pairs, cls=None, adds=None):
     C = dC.to_host()
     out = [C[int(o):int(o) + int(n)].copy() for o, n in zip(c_off, cl)]
     return out, m.value
+
+
+class Batch:
+    """A batch of products on device words captured once into a hipGraph
+    (bcg_batch_create) and replayed with run(); offsets/lengths in words."""
+
+    def __init__(self, dC, dA, dB, c_off, cl, a_off, al, b_off, bl, dD=None, d_off=None, dl=None):
+        self._keep = [np.ascontiguousarray(x, dtype=np.uint64) for x in (c_off, cl, a_off, al, b_off, bl)]
+        if dD is not None:
+            self._keep += [np.ascontiguousarray(d_off, np.uint64), np.ascontiguousarray(dl, np.uint64)]
+        ptrs = [k.ctypes.data_as(ctypes.POINTER(_sz)) for k in self._keep]
+        if dD is None:
+            ptrs += [None, None]
+        h = _vp()
+        check(lib().bcg_batch_create(ctypes.byref(h), dC.ptr, dA.ptr, dB.ptr, dD.ptr if dD is not None else None,
+                                     *ptrs, len(self._keep[0])), "bcg_batch_create")
+        self.handle = h.value
+
+    def run(self):
+        """Replay the graph; returns the worst rounding margin."""
+        m = ctypes.c_double()
+        check(lib().bcg_batch_run(self.handle, None, ctypes.byref(m)), "bcg_batch_run")
+        return m.value
+
+    def close(self):
+        if self.handle:
+            check(lib().bcg_batch_destroy(self.handle), "bcg_batch_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def muladd(a, b, d, cl=None):

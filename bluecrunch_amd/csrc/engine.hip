@@ -42,9 +42,10 @@ struct Engine {
   bool ready = false;
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t cap = nullptr;                     // graph capture of batches
   std::map<unsigned long long, RootsDev> roots;  // keyed by N
   std::map<int, cd *> ltab;                      // e^{2 pi i t/L}, keyed by L
-  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, margins, counters, bbufA, bbufB, bblk, bdesc, tdesc;
+  DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, counters;
   bool prof = false;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> free_events;
@@ -98,6 +99,7 @@ int init_locked(int device) {
   if (device < 0 || device >= n) return fail(BCG_E_ARG, "device index out of range");
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&g.cap, hipStreamNonBlocking));
   g.device = device;
   g.ready = true;
   if (int rc = ensure(g.margin, sizeof(unsigned int) * MARGIN_SLOTS * MARGIN_STRIDE)) return rc;
@@ -823,14 +825,31 @@ int mul_onepass(const Plan &p, uint32_t *C, size_t CL, const uint32_t *A, size_t
 // Batched multi-pass products of one plan (same N and pass split): every pass
 // is one launch over (tiles x products); product p owns arrays p*N in the
 // batch buffers and margin slot p. Descriptors already on the device.
+// Scratch of batched launches: descriptors, margins, class counters, the
+// batched transform arrays and carry block functions. The plain batch call
+// uses the engine's set; a captured batch owns one (its graph keeps the
+// addresses).
+struct BatchBufs {
+  DevBuf tdesc, mdesc, cdesc, margins, counters, tA, tB, blk;
+  void release() {
+    for (DevBuf *b : {&tdesc, &mdesc, &cdesc, &margins, &counters, &tA, &tB, &blk}) {
+      if (b->p) (void)hipFree(b->p);
+      b->p = nullptr;
+      b->bytes = 0;
+    }
+  }
+};
+
+BatchBufs g_bb;  // scratch of the plain batch call
+
 int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<MulDesc> &hdesc,
-                        unsigned int *dmargins, hipStream_t s) {
+                        unsigned int *dmargins, hipStream_t s, BatchBufs &bb) {
   const int count = (int)hdesc.size();
   const size_t arr = (size_t)p.N * sizeof(cd) * (size_t)count;
-  if (int rc = ensure(g.bbufA, arr)) return rc;
-  if (int rc = ensure(g.bbufB, arr)) return rc;
-  cd *da = (cd *)g.bbufA.p;
-  cd *db = (cd *)g.bbufB.p;
+  if (int rc = ensure(bb.tA, arr)) return rc;
+  if (int rc = ensure(bb.tB, arr)) return rc;
+  cd *da = (cd *)bb.tA.p;
+  cd *db = (cd *)bb.tB.p;
   PassArgs a;
   memset(&a, 0, sizeof a);
   a.desc = ddesc;
@@ -882,14 +901,14 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
     if (nb > nblk_max) nblk_max = nb;
   }
   nblk_max = (nblk_max + 15) & ~15ull;  // 16-byte aligned rows of block functions
-  if (int rc = ensure(g.bblk, nblk_max * (size_t)count)) return rc;
+  if (int rc = ensure(bb.blk, nblk_max * (size_t)count)) return rc;
   CarryArgs c;
   memset(&c, 0, sizeof c);
   c.desc = ddesc;
   c.coef_stride = 2 * p.N;
   c.coef = (const long long *)db;
   c.ncoef = 2 * p.N;
-  c.blk_fn = (unsigned char *)g.bblk.p;
+  c.blk_fn = (unsigned char *)bb.blk.p;
   c.nblk = nblk_max;
   c.wpb = (unsigned)wpb;
   {
@@ -1252,8 +1271,7 @@ int bcg_shutdown(void) {
   std::lock_guard<std::recursive_mutex> lk(g_lock);
   if (!g.ready) return 0;
   (void)hipStreamSynchronize(g.stream);
-  for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf, &g.desc, &g.margins,
-                    &g.counters, &g.bbufA, &g.bbufB, &g.bblk, &g.bdesc, &g.tdesc}) {
+  for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf, &g.desc, &g.counters}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->bytes = 0;
@@ -1269,7 +1287,10 @@ int bcg_shutdown(void) {
   for (auto e : g.free_events) (void)hipEventDestroy(e);
   g.free_events.clear();
   (void)hipStreamDestroy(g.stream);
+  (void)hipStreamDestroy(g.cap);
+  g_bb.release();
   g.stream = nullptr;
+  g.cap = nullptr;
   g.ready = false;
   return 0;
 }
@@ -1391,166 +1412,297 @@ int bcg_basecase_host(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, cons
   return 0;
 }
 
-int bcg_muladd_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const uint32_t *D, const size_t *c_off,
-                         const size_t *cl, const size_t *a_off, const size_t *al, const size_t *b_off, const size_t *bl,
-                         const size_t *d_off, const size_t *dl, size_t count, void *stream, double *worst_margin) {
-  ENTER();
-  hipStream_t s = pick(stream);
-  double worst = 0;
+}  // extern "C"
+
+namespace {
+
+// One batch call, planned on the host: tiny products (one wave each),
+// one-pass products grouped by length class (one persistent k_mul1 launch per
+// class), multi-pass products grouped by transform length (one launch per
+// pass, the product in grid.y), and the products that go alone (aliasing or
+// empty operands).
+struct BatchPlan {
+  std::vector<MulDesc> tiny, mdesc, cdesc;
+  std::vector<std::pair<Plan, std::pair<size_t, size_t>>> mgroups;  // plan, (first, count) in mdesc
+  std::vector<std::pair<int, std::pair<size_t, size_t>>> cgroups;   // L, (first, count) in cdesc
+  std::vector<size_t> morder, corder, single;                        // product index of each entry
+  std::vector<Plan> plans;
+  // the call's arguments (kept for the products re-run alone)
+  uint32_t *C = nullptr;
+  const uint32_t *A = nullptr, *B = nullptr, *D = nullptr;
+  std::vector<size_t> c_off, cl, a_off, al, b_off, bl, d_off, dl;
+};
+
+int batch_plan(BatchPlan &bp, uint32_t *C, const uint32_t *A, const uint32_t *B, const uint32_t *D,
+               const size_t *c_off, const size_t *cl, const size_t *a_off, const size_t *al, const size_t *b_off,
+               const size_t *bl, const size_t *d_off, const size_t *dl, size_t count) {
+  bp = BatchPlan();
+  bp.C = C;
+  bp.A = A;
+  bp.B = B;
+  bp.D = D;
+  bp.c_off.assign(c_off, c_off + count);
+  bp.cl.assign(cl, cl + count);
+  bp.a_off.assign(a_off, a_off + count);
+  bp.al.assign(al, al + count);
+  bp.b_off.assign(b_off, b_off + count);
+  bp.bl.assign(bl, bl + count);
+  bp.d_off.assign(count, 0);
+  bp.dl.assign(count, 0);
+  if (D && dl) {
+    for (size_t i = 0; i < count; i++) {
+      bp.dl[i] = dl[i];
+      bp.d_off[i] = d_off ? d_off[i] : 0;
+    }
+  }
+  bp.plans.assign(count, Plan());
   auto desc_of = [&](size_t i, const Plan *pl) {
     MulDesc d;
     d.a = A + a_off[i];
     d.b = B + b_off[i];
     d.c = C + c_off[i];
-    d.d = (D && dl && dl[i]) ? D + d_off[i] : nullptr;
+    d.d = bp.dl[i] ? D + bp.d_off[i] : nullptr;
     d.al = al[i];
     d.bl = bl[i];
     d.cl = cl[i];
-    d.dl = d.d ? dl[i] : 0;
+    d.dl = bp.dl[i];
     d.bits = pl ? pl->bits : 0;
     d.cbits = pl ? coef_bits(*pl) : 0;
     return d;
   };
-  // plan every product: tiny ones (schoolbook, one wave each), one-pass plans
-  // grouped by length class (persistent one-workgroup kernels), multi-pass
-  // plans grouped by transform length (one launch per pass), the rest singly
-  std::vector<size_t> tiny, single, retry;
   std::map<int, std::vector<size_t>> classes;
   std::map<unsigned long long, std::vector<size_t>> mclasses;
-  std::vector<Plan> plans(count);
   for (size_t i = 0; i < count; i++) {
     const uint32_t *pa = A + a_off[i];
     const uint32_t *pb = B + b_off[i];
     uint32_t *pc = C + c_off[i];
-    const uint32_t *pd = (D && dl && dl[i]) ? D + d_off[i] : nullptr;
+    const uint32_t *pd = bp.dl[i] ? D + bp.d_off[i] : nullptr;
     bool alias = (pc == pa || pc == pb);
     if (pd && pd == pc) return fail(BCG_E_ARG, "an addend may not alias its output");
     if (al[i] == 0 || bl[i] == 0 || cl[i] == 0) {
-      single.push_back(i);
+      bp.single.push_back(i);
       continue;
     }
     if (al[i] + bl[i] < 100) {
       if (alias || cl[i] > 4096)
-        single.push_back(i);
+        bp.single.push_back(i);
       else
-        tiny.push_back(i);
+        bp.tiny.push_back(desc_of(i, nullptr));
       continue;
     }
-    if (int rc = make_plan(al[i], bl[i], 0, plans[i])) return rc;
+    if (int rc = make_plan(al[i], bl[i], 0, bp.plans[i])) return rc;
     if (alias)
-      single.push_back(i);
-    else if (plans[i].P == 1)
-      classes[plans[i].L[0]].push_back(i);
+      bp.single.push_back(i);
+    else if (bp.plans[i].P == 1)
+      classes[bp.plans[i].L[0]].push_back(i);
     else
-      mclasses[plans[i].N].push_back(i);
+      mclasses[bp.plans[i].N].push_back(i);
   }
-  if (!tiny.empty()) {
-    if (int rc = ensure(g.tdesc, tiny.size() * sizeof(MulDesc))) return rc;
-    std::vector<MulDesc> hd;
-    hd.reserve(tiny.size());
-    for (size_t i : tiny) hd.push_back(desc_of(i, nullptr));
-    HIPCHK(hipMemcpyAsync(g.tdesc.p, hd.data(), hd.size() * sizeof(MulDesc), hipMemcpyHostToDevice, s));
-    {
-      ProfScope ps(BCG_K_OTHER, s);
-      LAUNCH(k_basecase_batch, (int)tiny.size(), 64, s, (const MulDesc *)g.tdesc.p, (int)tiny.size());
+  for (auto &kv : mclasses) {
+    const size_t first = bp.mdesc.size();
+    for (size_t i : kv.second) {
+      bp.mdesc.push_back(desc_of(i, &bp.plans[i]));
+      bp.morder.push_back(i);
     }
+    bp.mgroups.push_back({bp.plans[kv.second[0]], {first, kv.second.size()}});  // same N and pass split
+  }
+  for (auto &kv : classes) {
+    const size_t first = bp.cdesc.size();
+    for (size_t i : kv.second) {
+      bp.cdesc.push_back(desc_of(i, &bp.plans[i]));
+      bp.corder.push_back(i);
+    }
+    bp.cgroups.push_back({kv.first, {first, kv.second.size()}});
+  }
+  return 0;
+}
+
+// Descriptors to the device and scratch sized (no allocation afterwards).
+int batch_upload(const BatchPlan &bp, BatchBufs &bb, hipStream_t s) {
+  const size_t nm = bp.mdesc.size(), nc = bp.cdesc.size();
+  if (int rc = ensure(bb.tdesc, bp.tiny.size() * sizeof(MulDesc))) return rc;
+  if (int rc = ensure(bb.mdesc, nm * sizeof(MulDesc))) return rc;
+  if (int rc = ensure(bb.cdesc, nc * sizeof(MulDesc))) return rc;
+  if (int rc = ensure(bb.margins, (nm + nc) * sizeof(unsigned int))) return rc;
+  if (int rc = ensure(bb.counters, (bp.cgroups.size() + 1) * sizeof(int))) return rc;
+  size_t arr = 0, blk = 16;
+  for (auto &gr : bp.mgroups) {
+    const Plan &p = gr.first;
+    arr = std::max(arr, (size_t)p.N * sizeof(cd) * gr.second.second);
+    const unsigned long long wpb = (unsigned long long)CT * CWPT;
+    unsigned long long nb = 1;
+    for (size_t k = 0; k < gr.second.second; k++)
+      nb = std::max(nb, (unsigned long long)((bp.mdesc[gr.second.first + k].cl + wpb - 1) / wpb));
+    blk = std::max(blk, (size_t)((nb + 15) & ~15ull) * gr.second.second);
+  }
+  if (arr) {
+    if (int rc = ensure(bb.tA, arr)) return rc;
+    if (int rc = ensure(bb.tB, arr)) return rc;
+    if (int rc = ensure(bb.blk, blk)) return rc;
+  }
+  if (!bp.tiny.empty())
+    HIPCHK(hipMemcpyAsync(bb.tdesc.p, bp.tiny.data(), bp.tiny.size() * sizeof(MulDesc), hipMemcpyHostToDevice, s));
+  if (nm) HIPCHK(hipMemcpyAsync(bb.mdesc.p, bp.mdesc.data(), nm * sizeof(MulDesc), hipMemcpyHostToDevice, s));
+  if (nc) HIPCHK(hipMemcpyAsync(bb.cdesc.p, bp.cdesc.data(), nc * sizeof(MulDesc), hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+// Every batched launch of the plan, kernels only (capturable).
+int batch_launch(const BatchPlan &bp, BatchBufs &bb, hipStream_t s) {
+  const size_t nm = bp.mdesc.size(), nc = bp.cdesc.size();
+  unsigned int *mm = (unsigned int *)bb.margins.p, *cm = mm + nm;
+  if (nm + nc) LAUNCH(k_zero_u32, (int)((nm + nc + 255) / 256), 256, s, mm, (int)(nm + nc));
+  if (!bp.cgroups.empty()) LAUNCH(k_zero_i32, 1, 64, s, (int *)bb.counters.p, (int)bp.cgroups.size());
+  HIPCHK(hipGetLastError());
+  if (!bp.tiny.empty()) {
+    ProfScope ps(BCG_K_OTHER, s);
+    LAUNCH(k_basecase_batch, (int)bp.tiny.size(), 64, s, (const MulDesc *)bb.tdesc.p, (int)bp.tiny.size());
     HIPCHK(hipGetLastError());
   }
-  // multi-pass classes: all products of one transform length in one launch per pass
-  if (!mclasses.empty()) {
-    size_t nm = 0;
-    for (auto &kv : mclasses) nm += kv.second.size();
-    if (int rc = ensure(g.bdesc, nm * sizeof(MulDesc))) return rc;
-    if (int rc = ensure(g.margins, nm * sizeof(unsigned int))) return rc;
-    std::vector<MulDesc> all;
-    std::vector<size_t> order;
-    for (auto &kv : mclasses)
-      for (size_t i : kv.second) {
-        all.push_back(desc_of(i, &plans[i]));
-        order.push_back(i);
-      }
-    HIPCHK(hipMemcpyAsync(g.bdesc.p, all.data(), nm * sizeof(MulDesc), hipMemcpyHostToDevice, s));
-    LAUNCH(k_zero_u32, (int)((nm + 255) / 256), 256, s, (unsigned int *)g.margins.p, (int)nm);
-    size_t first = 0;
-    for (auto &kv : mclasses) {
-      std::vector<MulDesc> hd(all.begin() + first, all.begin() + first + kv.second.size());
-      Plan pc = plans[kv.second[0]];  // the products share N and the pass split
-      if (int rc = run_multipass_class(pc, (const MulDesc *)g.bdesc.p + first, hd,
-                                       (unsigned int *)g.margins.p + first, s))
-        return rc;
-      first += kv.second.size();
-    }
-    std::vector<unsigned int> mb(nm);
-    HIPCHK(hipMemcpyAsync(mb.data(), g.margins.p, nm * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+  for (auto &gr : bp.mgroups) {
+    const size_t first = gr.second.first, n = gr.second.second;
+    std::vector<MulDesc> hd(bp.mdesc.begin() + first, bp.mdesc.begin() + first + n);
+    if (int rc = run_multipass_class(gr.first, (const MulDesc *)bb.mdesc.p + first, hd, mm + first, s, bb)) return rc;
+  }
+  int ci = 0;
+  for (auto &gr : bp.cgroups) {
+    const size_t first = gr.second.first, n = gr.second.second;
+    if (int rc = run_class(gr.first, (const MulDesc *)bb.cdesc.p + first, (int)n, cm + first, (int *)bb.counters.p + ci,
+                           s))
+      return rc;
+    ci++;
+  }
+  return 0;
+}
+
+// Margins back; products at >= 0.25 re-run alone with b - 1; the products
+// that go alone run now.
+int batch_finish(const BatchPlan &bp, BatchBufs &bb, hipStream_t s, double *worst_margin) {
+  double worst = 0;
+  const size_t nm = bp.mdesc.size(), nc = bp.cdesc.size();
+  std::vector<size_t> retry;
+  if (nm + nc) {
+    std::vector<unsigned int> mb(nm + nc);
+    HIPCHK(hipMemcpyAsync(mb.data(), bb.margins.p, (nm + nc) * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    for (size_t k = 0; k < nm; k++) {
+    for (size_t k = 0; k < nm + nc; k++) {
       float f;
       memcpy(&f, &mb[k], sizeof f);
       if (f >= 0.25f)
-        retry.push_back(order[k]);
+        retry.push_back(k < nm ? bp.morder[k] : bp.corder[k - nm]);
       else if (f > worst)
         worst = f;
     }
   }
-  size_t nclass = 0;
-  for (auto &kv : classes) nclass += kv.second.size();
-  if (nclass) {
-    if (int rc = ensure(g.desc, nclass * sizeof(MulDesc))) return rc;
-    if (int rc = ensure(g.margins, nclass * sizeof(unsigned int))) return rc;
-    if (int rc = ensure(g.counters, (classes.size() + 1) * sizeof(int))) return rc;
-    std::vector<MulDesc> hd;
-    std::vector<size_t> order;
-    hd.reserve(nclass);
-    for (auto &kv : classes)
-      for (size_t i : kv.second) {
-        hd.push_back(desc_of(i, &plans[i]));
-        order.push_back(i);
-      }
-    HIPCHK(hipMemcpyAsync(g.desc.p, hd.data(), nclass * sizeof(MulDesc), hipMemcpyHostToDevice, s));
-    LAUNCH(k_zero_u32, (int)((nclass + 255) / 256), 256, s, (unsigned int *)g.margins.p, (int)nclass);
-    LAUNCH(k_zero_i32, 1, 64, s, (int *)g.counters.p, (int)classes.size());
-    size_t first = 0;
-    int ci = 0;
-    for (auto &kv : classes) {
-      int n = (int)kv.second.size();
-      if (int rc = run_class(kv.first, (const MulDesc *)g.desc.p + first, n, (unsigned int *)g.margins.p + first,
-                             (int *)g.counters.p + ci, s))
-        return rc;
-      first += (size_t)n;
-      ci++;
-    }
-    std::vector<unsigned int> mb(nclass);
-    HIPCHK(hipMemcpyAsync(mb.data(), g.margins.p, nclass * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    for (size_t k = 0; k < nclass; k++) {
-      float f;
-      memcpy(&f, &mb[k], sizeof f);
-      if (f >= 0.25f)
-        retry.push_back(order[k]);  // redo with a smaller digit size
-      else if (f > worst)
-        worst = f;
-    }
-  }
-  for (size_t i : single) {
+  auto one = [&](size_t i, int force_bits) -> int {
     bcg_mul_info inf;
-    const uint32_t *pd = (D && dl && dl[i]) ? D + d_off[i] : nullptr;
-    if (int rc = mul_dev_locked(C + c_off[i], cl[i], A + a_off[i], al[i], B + b_off[i], bl[i], 0, s, &inf, pd,
-                                pd ? dl[i] : 0))
+    const uint32_t *pd = bp.dl[i] ? bp.D + bp.d_off[i] : nullptr;
+    if (int rc = mul_dev_locked(bp.C + bp.c_off[i], bp.cl[i], bp.A + bp.a_off[i], bp.al[i], bp.B + bp.b_off[i],
+                                bp.bl[i], force_bits, s, &inf, pd, pd ? bp.dl[i] : 0))
       return rc;
     if (inf.margin > worst) worst = inf.margin;
-  }
+    return 0;
+  };
+  for (size_t i : bp.single)
+    if (int rc = one(i, 0)) return rc;
   for (size_t i : retry) {
-    bcg_mul_info inf;
-    int fb = plans[i].bits - 1;
+    int fb = bp.plans[i].bits - 1;
     if (fb < 4) return fail(BCG_E_MARGIN, "rounding margin >= 0.25 at the smallest digit size");
-    const uint32_t *pd = (D && dl && dl[i]) ? D + d_off[i] : nullptr;
-    if (int rc = mul_dev_locked(C + c_off[i], cl[i], A + a_off[i], al[i], B + b_off[i], bl[i], fb, s, &inf, pd,
-                                pd ? dl[i] : 0))
-      return rc;
-    if (inf.margin > worst) worst = inf.margin;
+    if (int rc = one(i, fb)) return rc;
   }
   HIPCHK(hipStreamSynchronize(s));
   if (worst_margin) *worst_margin = worst;
+  return 0;
+}
+
+}  // namespace
+
+// A batch captured once and replayed (see include/bluecrunch_gpu.h).
+struct bcg_batch_s {
+  BatchPlan plan;
+  BatchBufs bufs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int bcg_muladd_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, const uint32_t *D, const size_t *c_off,
+                         const size_t *cl, const size_t *a_off, const size_t *al, const size_t *b_off, const size_t *bl,
+                         const size_t *d_off, const size_t *dl, size_t count, void *stream, double *worst_margin) {
+  ENTER();
+  hipStream_t s = pick(stream);
+  BatchPlan bp;
+  if (int rc = batch_plan(bp, C, A, B, D, c_off, cl, a_off, al, b_off, bl, d_off, dl, count)) return rc;
+  if (int rc = batch_upload(bp, g_bb, s)) return rc;
+  if (int rc = batch_launch(bp, g_bb, s)) return rc;
+  return batch_finish(bp, g_bb, s, worst_margin);
+}
+
+int bcg_batch_create(bcg_batch *batch, uint32_t *C, const uint32_t *A, const uint32_t *B, const uint32_t *D,
+                     const size_t *c_off, const size_t *cl, const size_t *a_off, const size_t *al, const size_t *b_off,
+                     const size_t *bl, const size_t *d_off, const size_t *dl, size_t count) {
+  ENTER();
+  if (!batch) return fail(BCG_E_ARG, "null handle pointer");
+  *batch = nullptr;
+  bcg_batch_s *bt = new (std::nothrow) bcg_batch_s;
+  if (!bt) return fail(BCG_E_NOMEM, "batch handle");
+  hipStream_t s = g.stream;
+  int rc = batch_plan(bt->plan, C, A, B, D, c_off, cl, a_off, al, b_off, bl, d_off, dl, count);
+  if (!rc) rc = batch_upload(bt->plan, bt->bufs, s);
+  // one eager run: builds every table the kernels read, checks the launches
+  if (!rc) rc = batch_launch(bt->plan, bt->bufs, s);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = fail(BCG_E_HIP, "batch warm-up", e);
+  }
+  if (!rc) {
+    const bool prof = g.prof;
+    g.prof = false;  // no timing events inside the graph
+    hipError_t e = hipStreamBeginCapture(g.cap, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+      rc = fail(BCG_E_HIP, "begin capture", e);
+    } else {
+      int lrc = batch_launch(bt->plan, bt->bufs, g.cap);
+      hipGraph_t graph = nullptr;
+      e = hipStreamEndCapture(g.cap, &graph);
+      bt->graph = graph;
+      if (lrc) rc = lrc;
+      else if (e != hipSuccess || !graph) rc = fail(BCG_E_HIP, "end capture", e);
+    }
+    g.prof = prof;
+  }
+  if (!rc) {
+    hipError_t e = hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = fail(BCG_E_HIP, "graph instantiate", e);
+  }
+  if (rc) {
+    if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
+    if (bt->graph) (void)hipGraphDestroy(bt->graph);
+    bt->bufs.release();
+    delete bt;
+    return rc;
+  }
+  *batch = bt;
+  return 0;
+}
+
+int bcg_batch_run(bcg_batch batch, void *stream, double *worst_margin) {
+  ENTER();
+  if (!batch || !batch->exec) return fail(BCG_E_ARG, "invalid batch handle");
+  hipStream_t s = pick(stream);
+  HIPCHK(hipGraphLaunch(batch->exec, s));
+  return batch_finish(batch->plan, batch->bufs, s, worst_margin);
+}
+
+int bcg_batch_destroy(bcg_batch batch) {
+  ENTER();
+  if (!batch) return 0;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (batch->exec) (void)hipGraphExecDestroy(batch->exec);
+  if (batch->graph) (void)hipGraphDestroy(batch->graph);
+  batch->bufs.release();
+  delete batch;
   return 0;
 }
 

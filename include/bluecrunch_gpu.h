@@ -90,6 +90,23 @@ int bcg_muladd_batch_dev(uint32_t *C, const uint32_t *A, const uint32_t *B, cons
                          const size_t *cl, const size_t *a_off, const size_t *al, const size_t *b_off, const size_t *bl,
                          const size_t *d_off, const size_t *dl, size_t count, void *stream, double *worst_margin);
 
+/* A batch captured once and replayed (the many small products of a
+ * binary-splitting level or a Newton step, repeated): the products of one
+ * bcg_muladd_batch_dev call (same arguments; pointers and lengths fixed at
+ * creation) are planned, their descriptors uploaded, and every kernel launch
+ * captured into one hipGraph (creation also runs the batch once).
+ * bcg_batch_run replays the graph on stream (NULL: the library stream), reads
+ * the rounding margins back, re-runs alone any product whose margin reached
+ * 0.25, and runs the products that cannot be batched (empty or aliasing
+ * operands) one call each. A, B, C, D must stay valid until
+ * bcg_batch_destroy. Replays of one batch must not overlap. */
+typedef struct bcg_batch_s *bcg_batch;
+int bcg_batch_create(bcg_batch *batch, uint32_t *C, const uint32_t *A, const uint32_t *B, const uint32_t *D,
+                     const size_t *c_off, const size_t *cl, const size_t *a_off, const size_t *al,
+                     const size_t *b_off, const size_t *bl, const size_t *d_off, const size_t *dl, size_t count);
+int bcg_batch_run(bcg_batch batch, void *stream, double *worst_margin);
+int bcg_batch_destroy(bcg_batch batch);
+
 /* Reference-compatible transforms on device arrays of interleaved complex
  * doubles (fft.h:15-23 semantics: forward e^{+2 pi i/N}, bit-reversed output,
  * unscaled; inverse takes bit-reversed input, natural output, unscaled). */

@@ -78,3 +78,47 @@ def test_muladd_batch_binary_splitting_shape(bc):
     for (a, b), d, c, cl in zip(pairs, adds, out, cls):
         want = (words_to_int(a) * words_to_int(b) + words_to_int(d)) % (1 << (32 * cl))
         assert words_to_int(c) == want
+
+
+def _device_batch(bc, pairs, adds=None):
+    al = np.array([len(a) for a, _ in pairs], dtype=np.uint64)
+    bl = np.array([len(b) for _, b in pairs], dtype=np.uint64)
+    cl = al + bl
+    offs = lambda n: np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.uint64)  # noqa: E731
+    dA = bc.DeviceWords.from_host(np.concatenate([a for a, _ in pairs]))
+    dB = bc.DeviceWords.from_host(np.concatenate([b for _, b in pairs]))
+    dC = bc.DeviceWords(int(cl.sum()))
+    extra = {}
+    if adds is not None:
+        dl = np.array([len(d) for d in adds], dtype=np.uint64)
+        extra = dict(dD=bc.DeviceWords.from_host(np.concatenate(adds)), d_off=offs(dl), dl=dl)
+    return dA, dB, dC, (offs(cl), cl, offs(al), al, offs(bl), bl), extra
+
+
+def test_batch_graph_replays(bc):
+    # one hipGraph for tiny, one-pass and multi-pass products (and an addend),
+    # replayed three times with eager library work and the output cleared in
+    # between; every replay exact
+    rng = SplitMix64(123)
+    pairs = [(operand(rng, n), operand(rng, m)) for n, m in
+             [(3, 5), (20, 30), (200, 150), (1000, 3000), (4000, 4000), (16384, 9000), (30000, 20000), (7, 90)]]
+    adds = [operand(rng, 5) for _ in pairs]
+    dA, dB, dC, (c_off, cl, a_off, al, b_off, bl), extra = _device_batch(bc, pairs, adds)
+    want = [exact_product(a, b) for a, b in pairs]
+    for (d, w, n) in zip(adds, want, cl):
+        x = int.from_bytes(w.astype("<u4").tobytes(), "little") + int.from_bytes(d.astype("<u4").tobytes(), "little")
+        w[:] = np.frombuffer((x % (1 << (32 * int(n)))).to_bytes(4 * int(n), "little"), dtype="<u4")
+    batch = bc.Batch(dC, dA, dB, c_off, cl, a_off, al, b_off, bl, **extra)
+    try:
+        for rep in range(3):
+            bc.check(bc.lib().bcg_memcpy_h2d(dC.ptr, np.zeros(dC.n, np.uint32).ctypes.data, 4 * dC.n), "clear")
+            worst = batch.run()
+            C = dC.to_host()
+            for i, w in enumerate(want):
+                assert np.array_equal(C[int(c_off[i]):int(c_off[i] + cl[i])], w), (rep, i)
+            assert worst < 0.25
+            # eager work between replays (other products through the same library)
+            x, y = operand(rng, 5000), operand(rng, 7000)
+            assert np.array_equal(bc.mul(x, y)[0], exact_product(x, y))
+    finally:
+        batch.close()

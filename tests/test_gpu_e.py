@@ -110,3 +110,11 @@ def test_e_digits_1e7_md5(bc, golden):
     txt, ms = bc.e_digits(10 ** 7, raw=True)
     assert len(txt) == 10 ** 7 + 2 and txt[:2] == b"2."
     assert hashlib.md5(txt).hexdigest() == golden["e_md5"]["10000000"]
+
+
+@pytest.mark.gpu
+def test_e_digits_1e9_md5(bc, golden):
+    # the largest configuration: 1e9 digits (products up to ~2^29 complex points)
+    import hashlib
+    txt, ms = bc.e_digits(10 ** 9, raw=True)
+    assert hashlib.md5(txt).hexdigest() == golden["e_md5"]["1000000000"]

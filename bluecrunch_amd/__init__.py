@@ -31,7 +31,7 @@ EXPORTS = {
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
-        "bcg_plan_force", "bcg_plan_current", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_prof_enable", "bcg_prof_read",
+        "bcg_plan_force", "bcg_plan_current", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_prof_enable", "bcg_prof_select", "bcg_prof_read",
         "bcg_prof_reset",
     ],
     "fft.h": [
@@ -126,6 +126,7 @@ def lib():
     L.bcg_e_digits.argtypes = [_sz, ctypes.c_char_p, _sz, _f64p]
     L.bcg_e_digits_ex.argtypes = [_sz, ctypes.c_char_p, _sz, _f64p, i]
     L.bcg_prof_enable.argtypes = [i]
+    L.bcg_prof_select.argtypes = [ctypes.c_uint]
     L.bcg_prof_read.argtypes = [i, ctypes.POINTER(ctypes.c_long), _f64p]
     # reference C API
     L.fft_length.argtypes = [_sz]; L.fft_length.restype = _sz
@@ -397,6 +398,16 @@ def autotune(al, bl, square=False, reps=2):
 
 def prof_enable(on=True):
     check(lib().bcg_prof_enable(1 if on else 0), "prof_enable")
+
+
+def prof_select(kinds=None):
+    """Time only these kinds (names or numbers) while profiling; None: all."""
+    mask = 0xFFFFFFFF
+    if kinds is not None:
+        mask = 0
+        for k in kinds:
+            mask |= 1 << (PROF_KINDS[k] if isinstance(k, str) else k)
+    check(lib().bcg_prof_select(mask), "prof_select")
 
 
 def prof_read(kind):

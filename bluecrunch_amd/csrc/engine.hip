@@ -49,6 +49,7 @@ struct Engine {
   bool prof = false;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> free_events;
+  unsigned prof_kinds = ~0u;  // kinds bracketed by events while profiling
   long prof_n[BCG_PROF_KINDS] = {0};
   double prof_ms[BCG_PROF_KINDS] = {0};
 };
@@ -133,7 +134,7 @@ struct ProfScope {
   hipStream_t s;
   hipEvent_t a = nullptr;
   ProfScope(int k, hipStream_t st) : kind(k), s(st) {
-    if (g.prof) {
+    if (g.prof && ((g.prof_kinds >> k) & 1u)) {
       a = get_event();
       (void)hipEventRecord(a, s);
     }
@@ -2032,6 +2033,12 @@ int bcg_prof_enable(int on) {
   ENTER();
   if (!on) prof_collect();
   g.prof = on != 0;
+  return 0;
+}
+
+int bcg_prof_select(unsigned kinds) {
+  ENTER();
+  g.prof_kinds = kinds;
   return 0;
 }
 

@@ -188,6 +188,9 @@ int bcg_e_bsr_dev(size_t terms, uint32_t **pq, size_t *pl, size_t *ql);
 #define BCG_K_CARRY 5     /* coefficient -> word carry resolve */
 #define BCG_K_OTHER 6
 int bcg_prof_enable(int on);
+/* Bit mask of the kinds bracketed while enabled (bit k = kind k; default all):
+ * timing one kind keeps the event overhead off the others. */
+int bcg_prof_select(unsigned kinds);
 int bcg_prof_read(int kind, long *launches, double *ms);
 int bcg_prof_reset(void);
 

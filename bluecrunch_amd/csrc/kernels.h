@@ -112,11 +112,12 @@ __device__ __forceinline__ void margin_update(unsigned int *m, double d) {
 // where adding 1.5 * 2^52 leaves round-to-nearest(x) in the low mantissa bits.
 __device__ __forceinline__ long long round_coef(double x, bool fast, double &worst) {
   if (fast) {
+    // no clamp at 0: a negative result means an error of at least 1/2 in a
+    // non-negative coefficient, outside the regime the margin check trusts
     const double magic = 6755399441055744.0;  // 1.5 * 2^52
     const double y = x + magic;
     worst = fmax(worst, fabs(x - (y - magic)));
-    const long long v = __double_as_longlong(y) - __double_as_longlong(magic);
-    return v > 0 ? v : 0;
+    return __double_as_longlong(y) - __double_as_longlong(magic);
   }
   worst = fmax(worst, fabs(x - rint(x)));
   const long long v = (long long)(x + 0.5);
@@ -263,6 +264,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   // (M >= 128, N >= 8192 in multi-pass plans), so a digit's bit offset inside
   // its segment depends only on its column.
   uint32_t dre[PACK ? 16 : 1], dim_[PACK ? 16 : 1];
+  bool upper = false;                                      // does the upper half hold digits?
   if constexpr (PACK) {
     uint32_t *lw = reinterpret_cast<uint32_t *>(lds);
     const unsigned b = (unsigned)a.bits;
@@ -271,7 +273,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
     const unsigned long long row_w = (a.M * b) >> 5;       // words per row of digits
     const unsigned long long half_w = (a.N * b) >> 5;      // words per half
     const unsigned long long w0 = (m0 * b) >> 5;
-    const bool upper = half_w < a.al;                      // does the upper half hold digits?
+    upper = half_w < a.al;
     const unsigned n = (upper ? 2u : 1u) * L * SW;
     // eight independent loads in flight per thread per batch
     for (unsigned e0 = 0; e0 < n; e0 += 8u * TC::T) {
@@ -317,8 +319,10 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
       unsigned long long row = (unsigned long long)(q + t * SUB0);
       if constexpr (PACK) {
         const int h = i + t * PER0;
-        cd v = make_double2((double)dre[h], (double)dim_[h]);
-        x[t] = cmul(v, cmul(wpack, w64(h)));
+        const cd w = cmul(wpack, w64(h));
+        const double re = (double)dre[h];
+        // (a_j + i a_{j+N}) w, with a_{j+N} = 0 throughout when the upper half is empty
+        x[t] = upper ? cmul(make_double2(re, (double)dim_[h]), w) : make_double2(re * w.x, re * w.y);
       } else {
         x[t] = base[row * a.M + m];
       }

@@ -2,6 +2,7 @@
 // and the extern "C" ABI declared in include/bluecrunch_gpu.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>

@@ -1007,7 +1007,9 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
 // inside the block). Thread t owns blocks [t per, t per + per) with per a
 // multiple of 16, so its function bytes arrive as 16-byte loads; the scan
 // across threads uses wave shuffles; the first words of the carried blocks
-// of a 16-block chunk are read together.
+// of a 16-block chunk are read together. Carried all-ones blocks (long runs
+// of 0xffffffff, as in products close to a power of two) are cleared with
+// stores only, so a run costs no chain of dependent loads.
 static __global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
   __shared__ unsigned wfn[16];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1041,13 +1043,28 @@ static __global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
   for (unsigned long long c0 = b0; c0 < b1; c0 += 16) {
     const uint4 v = *reinterpret_cast<const uint4 *>(a.blk_fn + c0);
     const unsigned wv[4] = {v.x, v.y, v.z, v.w};
-    unsigned carried = 0;
+    unsigned carried = 0, ones = 0;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       if (c0 + k < b1) {
         const unsigned f = (wv[k >> 2] >> (8 * (k & 3))) & 3u;
-        if (c) carried |= 1u << k;
+        if (c) {
+          // a carried block whose function propagates holds only 0xffffffff
+          // words: it becomes zero without being read
+          if (f == 2u)
+            ones |= 1u << k;
+          else
+            carried |= 1u << k;
+        }
         c = (f >> c) & 1u;
+      }
+    }
+    if (ones) {
+      for (int k = 0; k < 16; k++) {
+        if (!((ones >> k) & 1u)) continue;
+        unsigned long long w = (c0 + k) * wpb, wend = w + wpb;
+        if (wend > a.CL) wend = a.CL;
+        for (; w < wend; w++) a.out[w] = 0u;
       }
     }
     uint32_t x[16];

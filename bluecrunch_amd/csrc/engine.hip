@@ -1998,37 +1998,52 @@ size_t bcg_e_terms(size_t words) { return e_terms_of(words); }
 int bcg_e_bsr_dev(size_t terms, uint32_t **pq, size_t *pl, size_t *ql) {
   ENTER();
   if (!pq || !pl || !ql) return fail(BCG_E_ARG, "null pointer");
-  return bsr_device(terms, pq, pl, ql, g.stream);
+  const int rc = bsr_device(terms, pq, pl, ql, g.stream);
+  if (!rc) ep_detach(*pq);
+  ep_trim();
+  return rc;
 }
 
 int bcg_div_fraction_dev(const uint32_t *P, size_t pl, const uint32_t *Q, size_t ql, size_t nw, uint32_t **F) {
   ENTER();
   if (!P || !Q || !pl || !ql || !nw || !F) return fail(BCG_E_ARG, "bad arguments");
   DInt f;
-  if (int rc = d_divide_fraction(P, pl, Q, ql, nw, f, g.stream)) return rc;
-  HIPCHK(hipStreamSynchronize(g.stream));
-  *F = f.p;
-  return 0;
+  int rc = d_divide_fraction(P, pl, Q, ql, nw, f, g.stream);
+  if (!rc && hipStreamSynchronize(g.stream) != hipSuccess) rc = fail(BCG_E_HIP, "division");
+  if (!rc) {
+    ep_detach(f.p);
+    *F = f.p;
+  } else {
+    dfree(f);
+  }
+  ep_trim();
+  return rc;
 }
 
 int bcg_e_digits(size_t digits, char *out, size_t cap, double *ms) {
   ENTER();
   if (!out) return fail(BCG_E_ARG, "null output");
-  return e_digits(digits, out, cap, ms, g.stream);
+  const int rc = e_digits(digits, out, cap, ms, g.stream);
+  ep_trim();
+  return rc;
 }
 
 int bcg_e_digits_ex(size_t digits, char *out, size_t cap, double *ms, int flags) {
   ENTER();
   if (!out) return fail(BCG_E_ARG, "null pointer");
   if (flags & ~BCG_E_PER_MULTIPLY) return fail(BCG_E_ARG, "unknown flags");
-  return e_digits(digits, out, cap, ms, g.stream, (flags & BCG_E_PER_MULTIPLY) != 0);
+  const int rc = e_digits(digits, out, cap, ms, g.stream, (flags & BCG_E_PER_MULTIPLY) != 0);
+  ep_trim();
+  return rc;
 }
 
 int bcg_radix_dev(const uint32_t *F, size_t nf, size_t groups, uint32_t *digits) {
   ENTER();
   if (!F || !nf || !digits) return fail(BCG_E_ARG, "bad arguments");
   if (!groups) return 0;
-  return radix_convert(F, nf, groups, digits, g.stream);
+  const int rc = radix_convert(F, nf, groups, digits, g.stream);
+  ep_trim();
+  return rc;
 }
 
 int bcg_prof_enable(int on) {

@@ -268,20 +268,23 @@ unsigned long long class_ceil(unsigned long long n) {
   }
 }
 
-int ilog2u(unsigned long long v) {
-  int l = 0;
-  while ((1ull << (l + 1)) <= v) l++;
-  return l;
-}
+int ilog2u(unsigned long long v) { return v ? 63 - __builtin_clzll(v) : 0; }
 
 // Largest digit size whose worst rounding error stays well below 1/4 at
-// transform length N (unsigned digits, right-angle convolution).
+// transform length N (unsigned digits, right-angle convolution). Lengths of
+// the form 2^k and 3 * 2^k (every length class) are memoised: batch planning
+// asks for them once per candidate digit size of every product.
 int bmax_for(unsigned long long N) {
+  static int memo[2][64];
+  const int k = ilog2u(N);
+  const int t = N == (1ull << k) ? 0 : (k >= 1 && N == (3ull << (k - 1)) ? 1 : -1);
+  if (t >= 0 && memo[t][k]) return memo[t][k];
   double lg = std::log2((double)N);
   double b = (50.5 - lg - std::log2(lg)) / 2.0;
   int bi = (int)std::floor(b);
   if (bi > 20) bi = 20;
   if (bi < 4) bi = 4;
+  if (t >= 0) memo[t][k] = bi;
   return bi;
 }
 
@@ -442,11 +445,14 @@ int make_plan(unsigned long long AL, unsigned long long BL, int force_bits, Plan
   best = Plan();
   unsigned long long bestN = 0;
   int bestb = 0;
+  // N grows as b falls, so the first feasible b gives the shortest length;
+  // smaller b keeping that length are preferred (more margin)
   for (int b = 20; b >= 4; b--) {
     if (force_bits > 0 && b != force_bits) continue;
     unsigned long long dA = digits_of(AL, b), dB = digits_of(BL, b);
     unsigned long long need = dA + dB - 1;
     unsigned long long N = class_ceil((need + 1) / 2);
+    if (bestN && N > bestN) break;
     if (force_bits <= 0 && b > bmax_for(N)) continue;
     if (bestN == 0 || N < bestN || (N == bestN && b < bestb)) {
       bestN = N;
@@ -524,6 +530,7 @@ int run_carry(const long long *coef, unsigned long long ncoef, int bits, int cbi
     ProfScope ps(BCG_K_CARRY, s);
     hipLaunchKernelGGL(k_carry_local, dim3((unsigned)nblk), dim3(CT), c.lds_slots * sizeof(long long), s, c);
     LAUNCH(k_carry_fix, 1, 1024, s, c);
+    LAUNCH(k_carry_apply, (unsigned)nblk, 256, s, c);
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -567,6 +574,7 @@ int carry_of_plan(const Plan &p, uint32_t *out, unsigned long long CL, hipStream
     ProfScope ps(BCG_K_CARRY, s);
     hipLaunchKernelGGL(k_carry_groups, dim3((unsigned)nblk), dim3(GCT), glds, s, c);
     LAUNCH(k_carry_fix, 1, 1024, s, c);
+    LAUNCH(k_carry_apply, (unsigned)nblk, 256, s, c);
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -927,6 +935,7 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
     hipLaunchKernelGGL(k_carry_local, dim3((unsigned)nblk_max, (unsigned)count), dim3(CT),
                        c.lds_slots * sizeof(long long), s, c);
     LAUNCH(k_carry_fix, count, 1024, s, c);
+    LAUNCH2(k_carry_apply, (unsigned)nblk_max, (unsigned)count, 256, s, c);
   }
   HIPCHK(hipGetLastError());
   return 0;

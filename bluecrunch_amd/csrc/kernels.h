@@ -1007,9 +1007,9 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
 // inside the block). Thread t owns blocks [t per, t per + per) with per a
 // multiple of 16, so its function bytes arrive as 16-byte loads; the scan
 // across threads uses wave shuffles; the first words of the carried blocks
-// of a 16-block chunk are read together. Carried all-ones blocks (long runs
-// of 0xffffffff, as in products close to a power of two) are cleared with
-// stores only, so a run costs no chain of dependent loads.
+// of a 16-block chunk are read together and take the +1 when it stops there;
+// the other carried blocks (all-ones blocks, as in products close to a power
+// of two, and +1s that run on) are marked for k_carry_apply.
 static __global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
   __shared__ unsigned wfn[16];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1043,50 +1043,67 @@ static __global__ __launch_bounds__(1024) void k_carry_fix(CarryArgs a) {
   for (unsigned long long c0 = b0; c0 < b1; c0 += 16) {
     const uint4 v = *reinterpret_cast<const uint4 *>(a.blk_fn + c0);
     const unsigned wv[4] = {v.x, v.y, v.z, v.w};
-    unsigned carried = 0, ones = 0;
+    unsigned carried = 0, heavy = 0;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       if (c0 + k < b1) {
         const unsigned f = (wv[k >> 2] >> (8 * (k & 3))) & 3u;
         if (c) {
           // a carried block whose function propagates holds only 0xffffffff
-          // words: it becomes zero without being read
+          // words (it becomes zero): left to k_carry_apply
           if (f == 2u)
-            ones |= 1u << k;
+            heavy |= 1u << k;
           else
             carried |= 1u << k;
         }
         c = (f >> c) & 1u;
       }
     }
-    if (ones) {
-      for (int k = 0; k < 16; k++) {
-        if (!((ones >> k) & 1u)) continue;
-        unsigned long long w = (c0 + k) * wpb, wend = w + wpb;
-        if (wend > a.CL) wend = a.CL;
-        for (; w < wend; w++) a.out[w] = 0u;
-      }
-    }
     uint32_t x[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const unsigned long long w = (c0 + k) * wpb;
-      x[k] = ((carried >> k) & 1u) && w < a.CL ? a.out[w] : 0u;
-    }
+    for (int k = 0; k < 16; k++) x[k] = ((carried >> k) & 1u) ? a.out[(c0 + k) * wpb] : 0u;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-      unsigned long long w = (c0 + k) * wpb;
-      if (!((carried >> k) & 1u) || w >= a.CL) continue;
-      uint32_t y = x[k] + 1u;
-      a.out[w] = y;
-      // rare: the +1 runs through 0xffffffff words
-      const unsigned long long wend = (c0 + k + 1) * wpb;
-      while (y == 0u && ++w < wend && w < a.CL) {
-        y = a.out[w] + 1u;
-        a.out[w] = y;
-      }
+      if (!((carried >> k) & 1u)) continue;
+      if (x[k] != 0xffffffffu)
+        a.out[(c0 + k) * wpb] = x[k] + 1u;
+      else
+        heavy |= 1u << k;  // the +1 runs on through 0xffffffff words
     }
+    if (heavy)
+      for (int k = 0; k < 16; k++)
+        if ((heavy >> k) & 1u) a.blk_fn[c0 + k] |= 4u;
   }
+}
+
+// The carried blocks k_carry_fix marked (bit 2 of the block function): one
+// workgroup per block adds the carry with all its threads, clearing a block of
+// 0xffffffff words, or the 0xffffffff words below the first other word, which
+// takes the +1 (a block whose function does not propagate has such a word).
+static __global__ __launch_bounds__(256) void k_carry_apply(CarryArgs a) {
+  __shared__ unsigned long long first;
+  const int tid = threadIdx.x;
+  bind_carry(a, blockIdx.y);
+  if (blockIdx.x >= a.nblk) return;
+  const unsigned f = a.blk_fn[blockIdx.x];
+  if (!(f & 4u)) return;
+  const unsigned long long w0 = (unsigned long long)blockIdx.x * a.wpb;
+  const unsigned long long w1 = w0 + a.wpb < a.CL ? w0 + a.wpb : a.CL;
+  if ((f & 3u) == 2u) {
+    for (unsigned long long w = w0 + tid; w < w1; w += 256) a.out[w] = 0u;
+    return;
+  }
+  if (tid == 0) first = w1;
+  __syncthreads();
+  for (unsigned long long w = w0 + tid; w < w1; w += 256)
+    if (a.out[w] != 0xffffffffu) {
+      atomicMin(&first, w);
+      break;
+    }
+  __syncthreads();
+  const unsigned long long fw = first;
+  for (unsigned long long w = w0 + tid; w < fw; w += 256) a.out[w] = 0u;
+  if (tid == 0 && fw < w1) a.out[fw] += 1u;
 }
 
 // --------------------------------------------------------------- utilities

@@ -202,3 +202,41 @@ def test_autotune_keeps_exactness(bc):
     a, b = operands(4, 300000, 300000)
     c, _ = bc.mul(a, b)
     assert product_fingerprint_ok(a, b, c)
+
+
+@pytest.mark.parametrize("k", [1000, 5000, 70000, 1 << 18])
+def test_carry_through_all_ones_blocks(bc, k):
+    # (2^(32k) - 1)(2^(32k) + 1) + 1 = 2^(64k): before the final +1 every word
+    # is 0xffffffff, so the carry enters whole carry blocks that propagate
+    # (they become zero) and stops in the top block after a run of ones
+    a = np.full(k, 0xffffffff, dtype=np.uint32)
+    b = np.zeros(k + 1, dtype=np.uint32)
+    b[0] = b[k] = 1
+    c, info = bc.muladd(a, b, np.array([1], dtype=np.uint32), 2 * k + 2)
+    want = np.zeros(2 * k + 2, dtype=np.uint32)
+    want[2 * k] = 1
+    assert np.array_equal(c, want), int(np.flatnonzero(c != want)[0])
+    assert info["margin"] < 0.25
+    # (2^(32k) - 2)(2^(32k) + 1) + 2 = 2^(64k) - 2^(32k): the carry out of the
+    # low word runs through the k - 1 words of ones above it, stops at word k
+    a[0] = 0xfffffffe
+    c, _ = bc.muladd(a, b, np.array([2], dtype=np.uint32), 2 * k + 2)
+    x = ((1 << (32 * k)) - 2) * ((1 << (32 * k)) + 1) + 2
+    assert words_to_int(c) == x
+
+
+def test_batched_carry_through_all_ones_blocks(bc):
+    pairs, adds, want = [], [], []
+    for k in (300, 2000, 9000):
+        a = np.full(k, 0xffffffff, dtype=np.uint32)
+        b = np.zeros(k + 1, dtype=np.uint32)
+        b[0] = b[k] = 1
+        pairs.append((a, b))
+        adds.append(np.array([1], dtype=np.uint32))
+        w = np.zeros(2 * k + 1, dtype=np.uint32)
+        w[2 * k] = 1
+        want.append(w)
+    out, margin = bc.mul_batch(pairs, adds=adds)
+    for c, w in zip(out, want):
+        assert np.array_equal(c, w)
+    assert margin < 0.25

@@ -76,6 +76,8 @@ int fail(int code, const char *what, hipError_t e = hipSuccess) {
     if (e_ != hipSuccess) return fail(BCG_E_HIP, #x, e_); \
   } while (0)
 
+void ep_trim(size_t keep = 0);  // idle blocks of the e pipeline's pool
+
 int ensure(DevBuf &b, size_t bytes) {
   if (b.bytes >= bytes) return 0;
   if (b.p) {
@@ -86,11 +88,91 @@ int ensure(DevBuf &b, size_t bytes) {
   }
   size_t want = bytes < 256 ? 256 : bytes;
   hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {  // give the pool's idle blocks back and try once more
+    (void)hipGetLastError();
+    ep_trim();
+    e = hipMalloc(&b.p, want);
+  }
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return fail(BCG_E_NOMEM, "device allocation failed", e);
   }
   b.bytes = want;
+  return 0;
+}
+
+// Pinned host staging for the library's own host<->device transfers
+// (descriptors, lengths, margins, small operands): a pageable copy goes
+// through a driver bounce buffer and can cost milliseconds on a host range the
+// driver has not seen; through this ring it is one DMA. Regions are handed out
+// in order; when the ring wraps, the device is drained first, so no queued
+// transfer still reads a region that is handed out again.
+struct HostRing {
+  char *p = nullptr;
+  size_t bytes = 0, pos = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+HostRing g_ring;
+
+int ring_take(size_t n, char **out) {
+  n = (n + 255) & ~(size_t)255;
+  if (g_ring.pos + n > g_ring.bytes) {
+    if (g_ring.p) HIPCHK(hipDeviceSynchronize());
+    g_ring.pos = 0;
+    if (n > g_ring.bytes) {
+      if (g_ring.p) HIPCHK(hipHostFree(g_ring.p));
+      g_ring.p = nullptr;
+      g_ring.bytes = 0;
+      const size_t want = n > ((size_t)16 << 20) ? n : ((size_t)16 << 20);
+      HIPCHK(hipHostMalloc((void **)&g_ring.p, want, hipHostMallocDefault));
+      g_ring.bytes = want;
+    }
+  }
+  *out = g_ring.p + g_ring.pos;
+  g_ring.pos += n;
+  return 0;
+}
+
+// Host to device, queued on s (the host data is copied before returning).
+int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  if (!bytes) return 0;
+  char *st;
+  if (int rc = ring_take(bytes, &st)) return rc;
+  memcpy(st, src, bytes);
+  HIPCHK(hipMemcpyAsync(dst, st, bytes, hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+// Device to host after the work queued on s; returns when dst holds the data.
+// Large copies go in 8 MiB chunks through two ring regions, the host copy of
+// one chunk overlapping the DMA of the next.
+int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  constexpr size_t CH = (size_t)8 << 20;
+  if (bytes <= CH) {
+    char *st;
+    if (int rc = ring_take(bytes ? bytes : 1, &st)) return rc;
+    if (bytes) HIPCHK(hipMemcpyAsync(st, src, bytes, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    memcpy(dst, st, bytes);
+    return 0;
+  }
+  for (int i = 0; i < 2; i++)
+    if (!g_ring.ev[i]) HIPCHK(hipEventCreateWithFlags(&g_ring.ev[i], hipEventDisableTiming));
+  char *st;
+  if (int rc = ring_take(2 * CH, &st)) return rc;
+  const size_t nch = (bytes + CH - 1) / CH;
+  for (size_t i = 0; i <= nch; i++) {
+    if (i < nch) {  // chunk i into region i & 1 (chunk i - 2 left it at step i - 1)
+      const size_t n = bytes - i * CH < CH ? bytes - i * CH : CH;
+      HIPCHK(hipMemcpyAsync(st + (i & 1) * CH, (const char *)src + i * CH, n, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipEventRecord(g_ring.ev[i & 1], s));
+    }
+    if (i >= 1) {  // chunk i - 1 out of its region while chunk i is in flight
+      const size_t j = i - 1, n = bytes - j * CH < CH ? bytes - j * CH : CH;
+      HIPCHK(hipEventSynchronize(g_ring.ev[j & 1]));
+      memcpy((char *)dst + j * CH, st + (j & 1) * CH, n);
+    }
+  }
   return 0;
 }
 
@@ -588,8 +670,7 @@ int zero_margin(hipStream_t s) {
 
 int read_margin(hipStream_t s, double *m) {
   unsigned int slots[MARGIN_SLOTS * MARGIN_STRIDE];
-  HIPCHK(hipMemcpyAsync(slots, g.margin.p, sizeof slots, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  if (int rc = d2h(slots, g.margin.p, sizeof slots, s)) return rc;
   unsigned int bits = 0;
   for (int i = 0; i < MARGIN_SLOTS; i++) bits = slots[i * MARGIN_STRIDE] > bits ? slots[i * MARGIN_STRIDE] : bits;
   float f;
@@ -825,7 +906,7 @@ int mul_onepass(const Plan &p, uint32_t *C, size_t CL, const uint32_t *A, size_t
   d.dl = D ? DL : 0;
   d.bits = p.bits;
   d.cbits = coef_bits(p);
-  HIPCHK(hipMemcpyAsync(g.desc.p, &d, sizeof d, hipMemcpyHostToDevice, s));
+  if (int rc = h2d(g.desc.p, &d, sizeof d, s)) return rc;
   if (int rc = zero_margin(s)) return rc;
   LAUNCH(k_zero_i32, 1, 64, s, (int *)g.counters.p, 1);
   if (int rc = run_class(p.L[0], (const MulDesc *)g.desc.p, 1, (unsigned int *)g.margin.p, (int *)g.counters.p, s))
@@ -1301,6 +1382,11 @@ int bcg_shutdown(void) {
   (void)hipStreamDestroy(g.stream);
   (void)hipStreamDestroy(g.cap);
   g_bb.release();
+  ep_trim();
+  if (g_ring.p) (void)hipHostFree(g_ring.p);
+  for (auto &e : g_ring.ev)
+    if (e) (void)hipEventDestroy(e);
+  g_ring = HostRing();
   g.stream = nullptr;
   g.cap = nullptr;
   g.ready = false;
@@ -1553,9 +1639,11 @@ int batch_upload(const BatchPlan &bp, BatchBufs &bb, hipStream_t s) {
     if (int rc = ensure(bb.blk, blk)) return rc;
   }
   if (!bp.tiny.empty())
-    HIPCHK(hipMemcpyAsync(bb.tdesc.p, bp.tiny.data(), bp.tiny.size() * sizeof(MulDesc), hipMemcpyHostToDevice, s));
-  if (nm) HIPCHK(hipMemcpyAsync(bb.mdesc.p, bp.mdesc.data(), nm * sizeof(MulDesc), hipMemcpyHostToDevice, s));
-  if (nc) HIPCHK(hipMemcpyAsync(bb.cdesc.p, bp.cdesc.data(), nc * sizeof(MulDesc), hipMemcpyHostToDevice, s));
+    if (int rc = h2d(bb.tdesc.p, bp.tiny.data(), bp.tiny.size() * sizeof(MulDesc), s)) return rc;
+  if (nm)
+    if (int rc = h2d(bb.mdesc.p, bp.mdesc.data(), nm * sizeof(MulDesc), s)) return rc;
+  if (nc)
+    if (int rc = h2d(bb.cdesc.p, bp.cdesc.data(), nc * sizeof(MulDesc), s)) return rc;
   return 0;
 }
 
@@ -1595,8 +1683,7 @@ int batch_finish(const BatchPlan &bp, BatchBufs &bb, hipStream_t s, double *wors
   std::vector<size_t> retry;
   if (nm + nc) {
     std::vector<unsigned int> mb(nm + nc);
-    HIPCHK(hipMemcpyAsync(mb.data(), bb.margins.p, (nm + nc) * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    if (int rc = d2h(mb.data(), bb.margins.p, (nm + nc) * sizeof(unsigned int), s)) return rc;
     for (size_t k = 0; k < nm + nc; k++) {
       float f;
       memcpy(&f, &mb[k], sizeof f);
@@ -2009,7 +2096,7 @@ int bcg_e_bsr_dev(size_t terms, uint32_t **pq, size_t *pl, size_t *ql) {
   if (!pq || !pl || !ql) return fail(BCG_E_ARG, "null pointer");
   const int rc = bsr_device(terms, pq, pl, ql, g.stream);
   if (!rc) ep_detach(*pq);
-  ep_trim();
+  ep_trim(EP_KEEP);
   return rc;
 }
 
@@ -2025,7 +2112,7 @@ int bcg_div_fraction_dev(const uint32_t *P, size_t pl, const uint32_t *Q, size_t
   } else {
     dfree(f);
   }
-  ep_trim();
+  ep_trim(EP_KEEP);
   return rc;
 }
 
@@ -2033,7 +2120,7 @@ int bcg_e_digits(size_t digits, char *out, size_t cap, double *ms) {
   ENTER();
   if (!out) return fail(BCG_E_ARG, "null output");
   const int rc = e_digits(digits, out, cap, ms, g.stream);
-  ep_trim();
+  ep_trim(EP_KEEP);
   return rc;
 }
 
@@ -2042,7 +2129,7 @@ int bcg_e_digits_ex(size_t digits, char *out, size_t cap, double *ms, int flags)
   if (!out) return fail(BCG_E_ARG, "null pointer");
   if (flags & ~BCG_E_PER_MULTIPLY) return fail(BCG_E_ARG, "unknown flags");
   const int rc = e_digits(digits, out, cap, ms, g.stream, (flags & BCG_E_PER_MULTIPLY) != 0);
-  ep_trim();
+  ep_trim(EP_KEEP);
   return rc;
 }
 
@@ -2051,7 +2138,7 @@ int bcg_radix_dev(const uint32_t *F, size_t nf, size_t groups, uint32_t *digits)
   if (!F || !nf || !digits) return fail(BCG_E_ARG, "bad arguments");
   if (!groups) return 0;
   const int rc = radix_convert(F, nf, groups, digits, g.stream);
-  ep_trim();
+  ep_trim(EP_KEEP);
   return rc;
 }
 

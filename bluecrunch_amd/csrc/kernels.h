@@ -210,6 +210,28 @@ __device__ __forceinline__ void write_group_sums(const PassArgs &a, cd *lds, uns
   }
 }
 
+// The same for one half (0: real parts, digits j0 + [0, C); 1: imaginary
+// parts, j0 + N + [0, C)) from 8-byte coefficient slots.
+template <int L, int C>
+__device__ __forceinline__ void write_group_half(const PassArgs &a, long long *lc, unsigned long long m0, int tid,
+                                                 int half) {
+  using TC = Tile<L, C>;
+  const int b = a.bits;
+  const int K = a.ygK;
+  for (int t = tid; t < L; t += TC::T) {
+    const int s0 = pad16(t * C);  // the row's C slots are contiguous
+    unsigned long long v[C];
+#pragma unroll
+    for (int i = 0; i < C; i++) v[i] = (unsigned long long)lc[s0 + i];
+    uint32_t *lw = reinterpret_cast<uint32_t *>(lc + s0);
+    const unsigned long long j0 = (unsigned long long)t * a.M + m0 + (half ? a.N : 0ull);
+    const int pos0 = (int)((j0 * (unsigned long long)b) & 31ull);
+    group_sum_words<C>(v, pos0, b, K, lw);
+    uint4 *dst = reinterpret_cast<uint4 *>(a.ybuf + (j0 / C) * (unsigned long long)K);
+    for (int q = 0; q < K / 4; q++) dst[q] = make_uint4(lw[4 * q], lw[4 * q + 1], lw[4 * q + 2], lw[4 * q + 3]);
+  }
+}
+
 // Whole-point form for single-pass plans (M = 1): j = row.
 __device__ __forceinline__ double unpack_point(const PassArgs &a, cd v, unsigned long long j) {
   return round_store(a, cmulc(v, omega_pow(a.roots, j)), j, 0, 0);
@@ -234,11 +256,54 @@ __device__ __forceinline__ void margin_reduce(unsigned int *m, double d, int mas
 // Length-L DIF down C adjacent columns of one span; stage 0 loads straight from
 // global (or from words when PACK), the rest run in LDS; the store multiplies
 // row r (frequency k) by the pass twiddle e^{2 pi i m k / S}.
+// Tiles of two radix-16 stages (L = 256): stage 1 runs in registers too, and
+// the transpose between the stages goes through LDS one half (real parts,
+// then imaginary parts) at a time, so a tile needs half the LDS and twice as
+// many waves fit on a CU.
+template <int L, int C>
+struct SplitTile {
+  static constexpr bool ON = !Tile<L, C>::THREE && Tile<L, C>::NST2 == 2 && Tile<L, C>::FIRSTB == 4;
+  static constexpr int BYTES = Tile<L, C>::LDS_ELEMS * (ON ? 8 : 16);
+};
+
+// Rows r0 + 16 t (t < 16) of column c in x become rows 16 r0 + t in y, one
+// double plane at a time through ld (barrier-separated; ends with a barrier).
+template <int C>
+__device__ __forceinline__ void split_transpose(double *ld, const cd (&x)[16], cd (&y)[16], int r0, int c,
+                                                bool to_blocks) {
+#pragma unroll
+  for (int plane = 0; plane < 2; plane++) {
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      const int row = to_blocks ? r0 + 16 * t : 16 * r0 + t;
+      ld[pad16(row * C + c)] = plane ? x[t].y : x[t].x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      const int row = to_blocks ? 16 * r0 + t : r0 + 16 * t;
+      if (plane)
+        y[t].y = ld[pad16(row * C + c)];
+      else
+        y[t].x = ld[pad16(row * C + c)];
+    }
+    __syncthreads();
+  }
+}
+
 template <int L, int C, bool PACK>
 __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   using TC = Tile<L, C>;
+  using ST = SplitTile<L, C>;
   bind_product(a);
-  __shared__ cd lds[TC::LDS_ELEMS];
+  // PACK stages the words under the tile (2 L segments of at most (20 C + 62) / 32 words)
+  constexpr int STAGE_BYTES = PACK ? 2 * L * ((C * 20 + 62) / 32) * 4 : 0;
+  // the split transpose pays off where compute dominates (the packing pass);
+  // the strided pass keeps the full tile (measured faster at fewer waves)
+  constexpr bool SPLIT = ST::ON && PACK;
+  constexpr int TILE_BYTES = SPLIT ? ST::BYTES : Tile<L, C>::LDS_ELEMS * 16;
+  __shared__ __align__(16) unsigned char lds_raw[TILE_BYTES > STAGE_BYTES ? TILE_BYTES : STAGE_BYTES];
+  cd *lds = reinterpret_cast<cd *>(lds_raw);
   const int tid = threadIdx.x;
   const int tile = tile_of_block(a.ntiles);
   const unsigned long long tpb = a.M / C;
@@ -310,6 +375,33 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
     }
     __syncthreads();
   }
+  if constexpr (SPLIT) {
+    // PER0 = 1, SUB0 = 16: rows r0 + 16 t, then rows 16 r0 + t after the transpose
+    cd x[16], y[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      if constexpr (PACK) {
+        const cd w = cmul(wpack, w64(t));
+        const double re = (double)dre[t];
+        x[t] = upper ? cmul(make_double2(re, (double)dim_[t]), w) : make_double2(re * w.x, re * w.y);
+      } else {
+        x[t] = base[(unsigned long long)(r0 + 16 * t) * a.M + m];
+      }
+    }
+    dif_levels<4>(x, a.tl, r0, 1);
+    split_transpose<C>(reinterpret_cast<double *>(lds_raw), x, y, r0, c, false);
+    dif_levels<4>(y, a.tl, 0, L / 16);
+    // row 16 r0 + t has frequency 16 brev4(t) + brev4(r0)
+    Walk16 w;
+    if (a.M > 1) w.template init_shared<C>(a.roots, a.tw_mul * m * brev((unsigned)r0, 4), a.tw_mul * m * 16ull);
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      cd v = y[t];
+      if (a.M > 1) v = cmul(v, w.at(t));
+      base[(unsigned long long)(16 * r0 + t) * a.M + m] = v;
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < PER0; i++) {
     int q = r0 + i * (TC::T / C);
@@ -362,11 +454,16 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
 
 // Mirror of k_fwd: staged load with the conjugate pass twiddle, DIT stages in
 // LDS, stage 0 stores straight to global, or (UNPACK) rounds to coefficients.
+// The unpacking pass of a split tile is held to 168 VGPRs (three waves per
+// SIMD; measured faster than two waves without the two spilled registers).
 template <int L, int C, bool UNPACK>
-__global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
+__global__ __launch_bounds__((Tile<L, C>::T))
+__attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3 : 1, 8))) void k_inv(PassArgs a) {
   using TC = Tile<L, C>;
+  using ST = SplitTile<L, C>;
   bind_product(a);
-  __shared__ cd lds[TC::LDS_ELEMS];
+  __shared__ __align__(16) unsigned char lds_raw[ST::BYTES];
+  cd *lds = reinterpret_cast<cd *>(lds_raw);
   const int tid = threadIdx.x;
   const int tile = tile_of_block(a.ntiles);
   const unsigned long long tpb = a.M / C;
@@ -377,6 +474,55 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_inv(PassArgs a) {
   const int c = tid % C;
   const int r0 = tid / C;
   const unsigned long long m = m0 + c;
+  if constexpr (ST::ON) {
+    // rows 16 r0 + t (frequency 16 brev4(t) + brev4(r0)) with the conjugate
+    // pass twiddle, stage 1 in registers, transpose to rows r0 + 16 t, stage 0
+    cd x[16], y[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) y[t] = base[(unsigned long long)(16 * r0 + t) * a.M + m];
+    Walk16 w;
+    if (a.M > 1) {
+      w.template init_shared<C>(a.roots, a.tw_mul * m * brev((unsigned)r0, 4), a.tw_mul * m * 16ull);
+#pragma unroll
+      for (int t = 0; t < 16; t++) y[t] = cmulc(y[t], w.at(t));
+    }
+    dit_levels<4>(y, a.tl, 0, L / 16);
+    double *ld = reinterpret_cast<double *>(lds_raw);
+    split_transpose<C>(ld, y, x, r0, c, true);
+    dit_levels<4>(x, a.tl, r0, 1);
+    if constexpr (!UNPACK) {
+#pragma unroll
+      for (int t = 0; t < 16; t++) base[(unsigned long long)(r0 + 16 * t) * a.M + m] = x[t];
+    } else {
+      double worst = 0.0;
+      const cd wun = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
+      if (a.ybuf) {
+        // group mode (coefficients below 2^51): the real coefficients, then
+        // the imaginary ones, each summed per row from LDS
+        long long *lc = reinterpret_cast<long long *>(lds_raw);
+        long long ci[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+          const cd f = cmulc(x[t], cmul(wun, w64(t)));
+          lc[pad16((r0 + 16 * t) * C + c)] = round_coef(f.x * a.inv_n, true, worst);
+          ci[t] = round_coef(f.y * a.inv_n, true, worst);
+        }
+        __syncthreads();
+        write_group_half<L, C>(a, lc, m0, tid, 0);
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 16; t++) lc[pad16((r0 + 16 * t) * C + c)] = ci[t];
+        __syncthreads();
+        write_group_half<L, C>(a, lc, m0, tid, 1);
+      } else {
+#pragma unroll
+        for (int t = 0; t < 16; t++)
+          worst = fmax(worst, round_store(a, cmulc(x[t], cmul(wun, w64(t))), (unsigned long long)(r0 + 16 * t), m, L));
+      }
+      margin_reduce(a.margin, worst, a.margin_mask);
+    }
+    return;
+  }
   if constexpr (TC::THREE) {
     for (int e = tid; e < TC::E; e += TC::T) {
       int row = e / C, cc = e % C;

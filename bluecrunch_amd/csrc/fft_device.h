@@ -257,6 +257,67 @@ __device__ __forceinline__ void dit_levels(cd (&x)[1 << RB], const cd *__restric
   }
 }
 
+// d e^{+2 pi i k / 16} and d e^{-2 pi i k / 16} for k in [0, 8), k a constant
+// after unrolling: the factors 1 and i are exact swaps, (+-1 + i)/sqrt 2 one
+// addition and one product per part, the rest a constant complex product.
+__device__ __forceinline__ cd mul_w16(cd d, int k) {
+  switch (k) {
+    case 0: return d;
+    case 2: return make_double2((d.x - d.y) * BCG_R2, (d.x + d.y) * BCG_R2);
+    case 4: return make_double2(-d.y, d.x);
+    case 6: return make_double2(-(d.x + d.y) * BCG_R2, (d.x - d.y) * BCG_R2);
+    default: return cmul(d, w16(k));
+  }
+}
+__device__ __forceinline__ cd mulc_w16(cd d, int k) {
+  switch (k) {
+    case 0: return d;
+    case 2: return make_double2((d.x + d.y) * BCG_R2, (d.y - d.x) * BCG_R2);
+    case 4: return make_double2(d.y, -d.x);
+    case 6: return make_double2((d.y - d.x) * BCG_R2, -(d.x + d.y) * BCG_R2);
+    default: return cmulc(d, w16(k));
+  }
+}
+
+// dif_levels / dit_levels for q = 0 (the innermost stage of a tile): every
+// twiddle is a sixteenth root of unity known at compile time.
+template <int RB>
+__device__ __forceinline__ void dif_levels_q0(cd (&x)[1 << RB]) {
+  constexpr int R = 1 << RB;
+#pragma unroll
+  for (int l = 0; l < RB; l++) {
+    const int half = R >> (l + 1);
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      if ((t & (2 * half - 1)) < half) {
+        const int j = t & (2 * half - 1);
+        cd a = x[t], b = x[t + half];
+        x[t] = cadd(a, b);
+        x[t + half] = mul_w16(csub(a, b), j * (16 / (2 * half)));
+      }
+    }
+  }
+}
+
+template <int RB>
+__device__ __forceinline__ void dit_levels_q0(cd (&x)[1 << RB]) {
+  constexpr int R = 1 << RB;
+#pragma unroll
+  for (int l = RB - 1; l >= 0; l--) {
+    const int half = R >> (l + 1);
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      if ((t & (2 * half - 1)) < half) {
+        const int j = t & (2 * half - 1);
+        cd a = x[t];
+        cd c = mulc_w16(x[t + half], j * (16 / (2 * half)));
+        x[t] = cadd(a, c);
+        x[t + half] = csub(a, c);
+      }
+    }
+  }
+}
+
 // 3-point DFT, natural order in and out, w = e^{+-2 pi i / 3}
 // (include/intrinsic.h:270-363 dft_3p / dft_3p_inv).
 template <bool INV>

@@ -390,7 +390,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
     }
     dif_levels<4>(x, a.tl, r0, 1);
     split_transpose<C>(reinterpret_cast<double *>(lds_raw), x, y, r0, c, false);
-    dif_levels<4>(y, a.tl, 0, L / 16);
+    dif_levels_q0<4>(y);
     // row 16 r0 + t has frequency 16 brev4(t) + brev4(r0)
     Walk16 w;
     if (a.M > 1) w.template init_shared<C>(a.roots, a.tw_mul * m * brev((unsigned)r0, 4), a.tw_mul * m * 16ull);
@@ -486,7 +486,7 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
 #pragma unroll
       for (int t = 0; t < 16; t++) y[t] = cmulc(y[t], w.at(t));
     }
-    dit_levels<4>(y, a.tl, 0, L / 16);
+    dit_levels_q0<4>(y);
     double *ld = reinterpret_cast<double *>(lds_raw);
     split_transpose<C>(ld, y, x, r0, c, true);
     dit_levels<4>(x, a.tl, r0, 1);
@@ -659,7 +659,7 @@ __device__ __forceinline__ void mid_forward(const PassArgs &a, cd *lds, const cd
       cd x[R];
 #pragma unroll
       for (int t = 0; t < R; t++) x[t] = lds[pad16(blk * R + t)];
-      dif_levels<RBL>(x, a.tl, 0, L / R);
+      dif_levels_q0<RBL>(x);
 #pragma unroll
       for (int t = 0; t < R; t++) xr[i * R + t] = x[t];
     }
@@ -712,7 +712,7 @@ __device__ __forceinline__ void mid_pipeline(const PassArgs &a, cd *lds, unsigne
         cd x[R];
 #pragma unroll
         for (int t = 0; t < R; t++) x[t] = xa[i * R + t];
-        dit_levels<RBL>(x, a.tl, 0, L / R);
+        dit_levels_q0<RBL>(x);
 #pragma unroll
         for (int t = 0; t < R; t++) lds[pad16(blk * R + t)] = x[t];
       }

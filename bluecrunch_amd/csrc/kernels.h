@@ -328,8 +328,9 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   // thread slices its 32 digits out of LDS. M b and N b are multiples of 32
   // (M >= 128, N >= 8192 in multi-pass plans), so a digit's bit offset inside
   // its segment depends only on its column.
-  uint32_t dre[PACK ? 16 : 1], dim_[PACK ? 16 : 1];
+  uint32_t dre[PACK && !SPLIT ? 16 : 1], dim_[PACK && !SPLIT ? 16 : 1];
   bool upper = false;                                      // does the upper half hold digits?
+  unsigned p_sw = 0, p_mask = 0, p_kk = 0, p_sh = 0;      // digit slicing (PACK)
   if constexpr (PACK) {
     uint32_t *lw = reinterpret_cast<uint32_t *>(lds);
     const unsigned b = (unsigned)a.bits;
@@ -361,19 +362,25 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
     const unsigned mask = (1u << b) - 1u;
     const unsigned off = (((unsigned)m0 * b) & 31u) + (unsigned)c * b;
     const unsigned kk = off >> 5, sh = off & 31u;
+    p_sw = SW;
+    p_mask = mask;
+    p_kk = kk;
+    p_sh = sh;
+    if constexpr (!SPLIT) {
 #pragma unroll
-    for (int h = 0; h < 16; h++) {
-      const unsigned row = (unsigned)r0 + (unsigned)h * (L / 16);
-      const unsigned k0 = row * SW + kk;
-      dre[h] = (uint32_t)((((unsigned long long)lw[k0 + 1] << 32) | lw[k0]) >> sh) & mask;
-      if (upper) {
-        const unsigned k1 = k0 + (unsigned)L * SW;
-        dim_[h] = (uint32_t)((((unsigned long long)lw[k1 + 1] << 32) | lw[k1]) >> sh) & mask;
-      } else {
-        dim_[h] = 0u;
+      for (int h = 0; h < 16; h++) {
+        const unsigned row = (unsigned)r0 + (unsigned)h * (L / 16);
+        const unsigned k0 = row * SW + kk;
+        dre[h] = (uint32_t)((((unsigned long long)lw[k0 + 1] << 32) | lw[k0]) >> sh) & mask;
+        if (upper) {
+          const unsigned k1 = k0 + (unsigned)L * SW;
+          dim_[h] = (uint32_t)((((unsigned long long)lw[k1 + 1] << 32) | lw[k1]) >> sh) & mask;
+        } else {
+          dim_[h] = 0u;
+        }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
   if constexpr (SPLIT) {
     // PER0 = 1, SUB0 = 16: rows r0 + 16 t, then rows 16 r0 + t after the transpose
@@ -381,13 +388,23 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
 #pragma unroll
     for (int t = 0; t < 16; t++) {
       if constexpr (PACK) {
+        // the digits come straight from the staged words (no digit registers)
+        const uint32_t *lw = reinterpret_cast<const uint32_t *>(lds);
+        const unsigned k0 = ((unsigned)r0 + 16u * (unsigned)t) * p_sw + p_kk;
         const cd w = cmul(wpack, w64(t));
-        const double re = (double)dre[t];
-        x[t] = upper ? cmul(make_double2(re, (double)dim_[t]), w) : make_double2(re * w.x, re * w.y);
+        const double re = (double)((uint32_t)((((unsigned long long)lw[k0 + 1] << 32) | lw[k0]) >> p_sh) & p_mask);
+        if (upper) {
+          const unsigned k1 = k0 + (unsigned)L * p_sw;
+          const double im = (double)((uint32_t)((((unsigned long long)lw[k1 + 1] << 32) | lw[k1]) >> p_sh) & p_mask);
+          x[t] = cmul(make_double2(re, im), w);
+        } else {
+          x[t] = make_double2(re * w.x, re * w.y);
+        }
       } else {
         x[t] = ld_nt(&base[(unsigned long long)(r0 + 16 * t) * a.M + m]);
       }
     }
+    if constexpr (PACK) __syncthreads();  // the transpose reuses the staging area
     dif_levels<4>(x, a.tl, r0, 1);
     split_transpose<C>(reinterpret_cast<double *>(lds_raw), x, y, r0, c, false);
     dif_levels_q0<4>(y);

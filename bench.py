@@ -95,13 +95,19 @@ def e_pipeline(bc, digits, compare_digits):
     with open(os.path.join(ROOT, "tests", "golden", "vectors.json")) as f:
         gold = json.load(f)["e_md5"]
     bc.e_digits(1000)  # warm-up (kernels, tables)
+    bc.stats_reset()
     t0 = time.perf_counter()
     text, ms = bc.e_digits(digits, raw=True)
     wall = time.perf_counter() - t0
+    st = bc.stats()
     md5 = hashlib.md5(text).hexdigest()
     out = {"digits": digits, "wall_s": wall, "bsr_ms": ms["bsr_ms"], "div_ms": ms["div_ms"],
            "radix_ms": ms["radix_ms"], "md5": md5,
-           "md5_ok": (md5 == gold[str(digits)]) if str(digits) in gold else None}
+           "md5_ok": (md5 == gold[str(digits)]) if str(digits) in gold else None,
+           # every FFT product of the run: the worst rounding margin, and the
+           # digit size and margin at the longest transform
+           "products": st["products"], "worst_margin": st["worst_margin"], "max_points": st["max_points"],
+           "bits_at_max": st["bits_at_max"], "margin_at_max": st["margin_at_max"]}
     if compare_digits:
         _, mb = bc.e_digits(compare_digits)
         _, mp = bc.e_digits(compare_digits, per_multiply=True)

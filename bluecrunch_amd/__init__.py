@@ -32,7 +32,7 @@ EXPORTS = {
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
         "bcg_plan_force", "bcg_plan_current", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_prof_enable", "bcg_prof_select", "bcg_prof_read",
-        "bcg_prof_reset",
+        "bcg_prof_reset", "bcg_stats_reset", "bcg_stats_get",
     ],
     "fft.h": [
         "twiddle_table", "twiddle_table_size", "fft_ensure_table", "table_select", "fft_free_table",
@@ -128,6 +128,7 @@ def lib():
     L.bcg_prof_enable.argtypes = [i]
     L.bcg_prof_select.argtypes = [ctypes.c_uint]
     L.bcg_prof_read.argtypes = [i, ctypes.POINTER(ctypes.c_long), _f64p]
+    L.bcg_stats_get.argtypes = [ctypes.POINTER(Stats)]
     # reference C API
     L.fft_length.argtypes = [_sz]; L.fft_length.restype = _sz
     L.table_select.argtypes = [_sz]; L.table_select.restype = _sz
@@ -419,6 +420,27 @@ def prof_read(kind):
 
 def prof_reset():
     check(lib().bcg_prof_reset(), "prof_reset")
+
+
+class Stats(ctypes.Structure):
+    """bcg_stats: running statistics over FFT products since the last reset."""
+    _fields_ = [("worst_margin", ctypes.c_double), ("products", ctypes.c_size_t), ("max_points", ctypes.c_size_t),
+                ("bits_at_max", ctypes.c_int), ("margin_at_max", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def stats_reset():
+    check(lib().bcg_stats_reset(), "stats_reset")
+
+
+def stats():
+    """Worst margin, product count and the digit size / margin at the longest
+    transform since the last stats_reset()."""
+    st = Stats()
+    check(lib().bcg_stats_get(ctypes.byref(st)), "stats_get")
+    return st.as_dict()
 
 
 def sync():

@@ -57,7 +57,20 @@ struct Engine {
 };
 
 Engine g;
+bcg_stats g_stats;  // see bcg_stats_get
 std::recursive_mutex g_lock;
+
+void note_product(unsigned long long points, int bits, double margin) {
+  g_stats.products++;
+  if (margin > g_stats.worst_margin) g_stats.worst_margin = margin;
+  if (points > g_stats.max_points) {
+    g_stats.max_points = points;
+    g_stats.bits_at_max = bits;
+    g_stats.margin_at_max = margin;
+  } else if (points == g_stats.max_points && margin > g_stats.margin_at_max) {
+    g_stats.margin_at_max = margin;
+  }
+}
 thread_local std::string g_err;
 
 int fail(int code, const char *what, hipError_t e = hipSuccess) {
@@ -1105,6 +1118,7 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
     }
     break;
   }
+  note_product(inf.points, inf.bits, inf.margin);
   if (info) *info = inf;
   return 0;
 }
@@ -1687,10 +1701,13 @@ int batch_finish(const BatchPlan &bp, BatchBufs &bb, hipStream_t s, double *wors
     for (size_t k = 0; k < nm + nc; k++) {
       float f;
       memcpy(&f, &mb[k], sizeof f);
-      if (f >= 0.25f)
-        retry.push_back(k < nm ? bp.morder[k] : bp.corder[k - nm]);
-      else if (f > worst)
-        worst = f;
+      const size_t i = k < nm ? bp.morder[k] : bp.corder[k - nm];
+      if (f >= 0.25f) {
+        retry.push_back(i);
+      } else {
+        if (f > worst) worst = f;
+        note_product(bp.plans[i].N, bp.plans[i].bits, f);
+      }
     }
   }
   auto one = [&](size_t i, int force_bits) -> int {
@@ -2146,6 +2163,19 @@ int bcg_prof_enable(int on) {
   ENTER();
   if (!on) prof_collect();
   g.prof = on != 0;
+  return 0;
+}
+
+int bcg_stats_reset(void) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  g_stats = bcg_stats();
+  return 0;
+}
+
+int bcg_stats_get(bcg_stats *out) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  if (!out) return fail(BCG_E_ARG, "null pointer");
+  *out = g_stats;
   return 0;
 }
 

@@ -194,6 +194,21 @@ int bcg_prof_select(unsigned kinds);
 int bcg_prof_read(int kind, long *launches, double *ms);
 int bcg_prof_reset(void);
 
+/* Running statistics over every FFT product since the last bcg_stats_reset
+ * (single, batched and those inside the e pipeline): the worst rounding
+ * margin, the product count, and the digit size and margin at the longest
+ * transform. No counterpart in the reference (its fft_to_int keeps no
+ * margin); for reports such as the e pipeline's largest transform. */
+typedef struct {
+  double worst_margin;      /* max |x - round(x)| over all products */
+  size_t products;          /* FFT products counted */
+  size_t max_points;        /* longest transform, complex points */
+  int bits_at_max;          /* digit size at that length */
+  double margin_at_max;     /* worst margin at that length */
+} bcg_stats;
+int bcg_stats_reset(void);
+int bcg_stats_get(bcg_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -240,3 +240,22 @@ def test_batched_carry_through_all_ones_blocks(bc):
     for c, w in zip(out, want):
         assert np.array_equal(c, w)
     assert margin < 0.25
+
+
+def test_running_stats(bc):
+    bc.stats_reset()
+    assert bc.stats()["products"] == 0
+    a, b = operands(5, 5000, 7000)
+    _, info = bc.mul(a, b)
+    st = bc.stats()
+    assert st["products"] == 1
+    assert st["max_points"] == info["points"] and st["bits_at_max"] == info["bits"]
+    assert st["margin_at_max"] == info["margin"] and st["worst_margin"] == info["margin"]
+    # a batch counts each FFT product; the tiny ones (schoolbook) are not FFT products
+    pairs = [operands(10 + k, 300 + 50 * k, 200) for k in range(5)] + [operands(99, 10, 20)]
+    _, worst = bc.mul_batch(pairs)
+    st = bc.stats()
+    assert st["products"] == 6
+    assert st["worst_margin"] >= worst
+    bc.stats_reset()
+    assert bc.stats()["max_points"] == 0

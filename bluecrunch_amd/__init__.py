@@ -565,10 +565,13 @@ def e_digits(digits, per_multiply=False, raw=False):
     """("2." + digits decimal digits of e, timings dict) computed on the GPU.
 
     per_multiply: binary-splitting products one call each instead of batched.
-    raw: return the bytes object (no decode) for very long outputs."""
-    buf = ctypes.create_string_buffer(digits + 3)
+    raw: return a read-only memoryview of the bytes (no zero fill, no copy,
+    no decode) for very long outputs."""
+    buf = np.empty(digits + 3, dtype=np.uint8)
     ms = (ctypes.c_double * 4)()
-    check(lib().bcg_e_digits_ex(digits, buf, digits + 3, ms, 1 if per_multiply else 0), "bcg_e_digits")
-    val = ctypes.string_at(buf, digits + 2)
+    check(lib().bcg_e_digits_ex(digits, buf.ctypes.data_as(ctypes.c_char_p), digits + 3, ms,
+                                1 if per_multiply else 0), "bcg_e_digits")
     times = {"bsr_ms": ms[0], "div_ms": ms[1], "radix_ms": ms[2], "total_ms": ms[3]}
-    return (val if raw else val.decode()), times
+    if raw:
+        return memoryview(buf)[:digits + 2].toreadonly(), times
+    return buf[:digits + 2].tobytes().decode(), times

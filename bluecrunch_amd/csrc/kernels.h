@@ -764,14 +764,20 @@ __device__ __forceinline__ void mid_pipeline(const PassArgs &a, cd *lds, unsigne
 }
 
 // Middle pass of a multi-pass plan (in place into a.data), or a whole one-pass
-// product rounded into a.coef when PACK/UNPACK.
-template <int L, bool SQUARE, bool PACK, bool UNPACK>
-__global__ __launch_bounds__((Tile<L, 1>::T)) void k_mid(PassArgs a) {
+// product rounded into a.coef when PACK/UNPACK. BPW contiguous blocks per
+// workgroup (each with its own LDS tile): blocks of L < 1024 (1536 for 3 2^k)
+// have fewer than 64 threads, and one block per workgroup would leave most
+// lanes of the wave idle.
+template <int L, bool SQUARE, bool PACK, bool UNPACK, int BPW = 1>
+__global__ __launch_bounds__((Tile<L, 1>::T * BPW)) void k_mid(PassArgs a) {
   using TC = Tile<L, 1>;
+  static_assert(BPW == 1 || !PACK, "one-pass products take one workgroup each");
   bind_product(a);
-  __shared__ cd lds[TC::LDS_ELEMS];
-  const int tid = threadIdx.x;
-  const unsigned long long blk0 = (unsigned long long)blockIdx.x * L;
+  __shared__ cd lds_all[TC::LDS_ELEMS * BPW];
+  const int sub = BPW > 1 ? (int)threadIdx.x / TC::T : 0;
+  const int tid = BPW > 1 ? (int)threadIdx.x % TC::T : (int)threadIdx.x;
+  cd *lds = lds_all + sub * TC::LDS_ELEMS;
+  const unsigned long long blk0 = ((unsigned long long)blockIdx.x * BPW + sub) * L;
   cd xa[TC::EPT];
   mid_pipeline<L, SQUARE, PACK>(a, lds, blk0, xa, tid);
   constexpr int R0 = 1 << TC::FIRSTB;

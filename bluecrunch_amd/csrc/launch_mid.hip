@@ -4,14 +4,23 @@
 
 namespace bcg {
 
+// grid = number of contiguous blocks of L points (1 for a one-pass product).
+// Middle passes with blocks of fewer than 64 threads put BPW = 64 / T blocks
+// in one workgroup (one full wave) when the block count allows it.
 template <int L>
 int launch_mid(bool sq, bool pack, int grid, int gy, hipStream_t s, const PassArgs &a) {
   constexpr int T = Tile<L, 1>::T;
+  constexpr int BPW = T < 64 ? 64 / T : 1;
   if (pack) {
     if (sq)
       BCG_LAUNCH2((k_mid<L, true, true, true>), grid, gy, T, s, a);
     else
       BCG_LAUNCH2((k_mid<L, false, true, true>), grid, gy, T, s, a);
+  } else if (BPW > 1 && grid % BPW == 0) {
+    if (sq)
+      BCG_LAUNCH2((k_mid<L, true, false, false, BPW>), grid / BPW, gy, T * BPW, s, a);
+    else
+      BCG_LAUNCH2((k_mid<L, false, false, false, BPW>), grid / BPW, gy, T * BPW, s, a);
   } else {
     if (sq)
       BCG_LAUNCH2((k_mid<L, true, false, false>), grid, gy, T, s, a);

@@ -45,6 +45,11 @@ struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t cap = nullptr;                     // graph capture of batches
+  // side streams of a batch: one-pass length classes run beside each other
+  // (forked from and joined back into the batch's stream by events)
+  static constexpr int NAUX = 3;
+  hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr, join_ev[NAUX] = {nullptr, nullptr, nullptr};
   std::map<unsigned long long, RootsDev> roots;  // keyed by N
   std::map<int, cd *> ltab;                      // e^{2 pi i t/L}, keyed by L
   DevBuf bufA, bufB, wA, wB, wC, blk, margin, zbuf, desc, counters;
@@ -198,6 +203,11 @@ int init_locked(int device) {
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&g.cap, hipStreamNonBlocking));
+  for (int i = 0; i < Engine::NAUX; i++) {
+    HIPCHK(hipStreamCreateWithFlags(&g.aux[i], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&g.join_ev[i], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&g.fork_ev, hipEventDisableTiming));
   g.device = device;
   g.ready = true;
   if (int rc = ensure(g.margin, sizeof(unsigned int) * MARGIN_SLOTS * MARGIN_STRIDE)) return rc;
@@ -1393,6 +1403,14 @@ int bcg_shutdown(void) {
   prof_collect();
   for (auto e : g.free_events) (void)hipEventDestroy(e);
   g.free_events.clear();
+  for (int i = 0; i < Engine::NAUX; i++) {
+    if (g.aux[i]) (void)hipStreamDestroy(g.aux[i]);
+    if (g.join_ev[i]) (void)hipEventDestroy(g.join_ev[i]);
+    g.aux[i] = nullptr;
+    g.join_ev[i] = nullptr;
+  }
+  if (g.fork_ev) (void)hipEventDestroy(g.fork_ev);
+  g.fork_ev = nullptr;
   (void)hipStreamDestroy(g.stream);
   (void)hipStreamDestroy(g.cap);
   g_bb.release();
@@ -1668,6 +1686,37 @@ int batch_launch(const BatchPlan &bp, BatchBufs &bb, hipStream_t s) {
   if (nm + nc) LAUNCH(k_zero_u32, (int)((nm + nc + 255) / 256), 256, s, mm, (int)(nm + nc));
   if (!bp.cgroups.empty()) LAUNCH(k_zero_i32, 1, 64, s, (int *)bb.counters.p, (int)bp.cgroups.size());
   HIPCHK(hipGetLastError());
+  // one-pass length classes: independent persistent launches, spread over the
+  // side streams (largest class first), beside the multi-pass groups and the
+  // tiny products on s, so that small classes do not leave the GPU idle one
+  // after another; joined back into s at the end
+  const bool fork = !bp.cgroups.empty();
+  if (fork) {
+    std::vector<int> order(bp.cgroups.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+      const double wx = (double)bp.cgroups[x].second.second * bp.cgroups[x].first;
+      const double wy = (double)bp.cgroups[y].second.second * bp.cgroups[y].first;
+      return wx > wy;
+    });
+    HIPCHK(hipEventRecord(g.fork_ev, s));
+    for (int i = 0; i < Engine::NAUX; i++) HIPCHK(hipStreamWaitEvent(g.aux[i], g.fork_ev, 0));
+    int rc = 0;
+    for (size_t k = 0; k < order.size() && !rc; k++) {
+      const int ci = order[k];
+      const auto &gr = bp.cgroups[ci];
+      const size_t first = gr.second.first, n = gr.second.second;
+      rc = run_class(gr.first, (const MulDesc *)bb.cdesc.p + first, (int)n, cm + first, (int *)bb.counters.p + ci,
+                     g.aux[k % Engine::NAUX]);
+    }
+    if (rc) {  // the side streams still rejoin s before the error returns
+      for (int i = 0; i < Engine::NAUX; i++) {
+        (void)hipEventRecord(g.join_ev[i], g.aux[i]);
+        (void)hipStreamWaitEvent(s, g.join_ev[i], 0);
+      }
+      return rc;
+    }
+  }
   if (!bp.tiny.empty()) {
     ProfScope ps(BCG_K_OTHER, s);
     LAUNCH(k_basecase_batch, (int)bp.tiny.size(), 64, s, (const MulDesc *)bb.tdesc.p, (int)bp.tiny.size());
@@ -1678,13 +1727,11 @@ int batch_launch(const BatchPlan &bp, BatchBufs &bb, hipStream_t s) {
     std::vector<MulDesc> hd(bp.mdesc.begin() + first, bp.mdesc.begin() + first + n);
     if (int rc = run_multipass_class(gr.first, (const MulDesc *)bb.mdesc.p + first, hd, mm + first, s, bb)) return rc;
   }
-  int ci = 0;
-  for (auto &gr : bp.cgroups) {
-    const size_t first = gr.second.first, n = gr.second.second;
-    if (int rc = run_class(gr.first, (const MulDesc *)bb.cdesc.p + first, (int)n, cm + first, (int *)bb.counters.p + ci,
-                           s))
-      return rc;
-    ci++;
+  if (fork) {
+    for (int i = 0; i < Engine::NAUX; i++) {
+      HIPCHK(hipEventRecord(g.join_ev[i], g.aux[i]));
+      HIPCHK(hipStreamWaitEvent(s, g.join_ev[i], 0));
+    }
   }
   return 0;
 }

@@ -431,6 +431,15 @@ bool fill_strides(Plan &p) {
 
 bool is_pow2(unsigned long long v) { return v && !(v & (v - 1)); }
 
+// Tile width of inverse pass i: a middle pass (i > 0) of 256-point tiles runs
+// 32 columns wide (its two-stage split tile still fits two workgroups per CU,
+// and 512-byte row runs measured faster than 256-byte ones); every other pass
+// uses the plan's width, the forward one included.
+int inv_cols(const Plan &p, int i) {
+  if (i > 0 && i < p.P - 1 && p.L[i] == 256 && p.C[i] == 16 && p.M[i] % 32 == 0) return 32;
+  return p.C[i];
+}
+
 // Adds p and, for each strided pass with 16-column tiles, the variant with
 // 8-column tiles (half the LDS per workgroup: more workgroups per CU).
 void push_with_tiles(std::vector<Plan> &out, const Plan &p) {
@@ -801,7 +810,9 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
       b.ybuf = (uint32_t *)g.bufB.p;
       b.ygK = group_stride(group_words(p));
     }
-    fwd_fn f = pick_inv(p.L[i], p.C[i]);
+    const int ci = inv_cols(p, i);
+    b.ntiles = (int)(p.N / ((unsigned long long)p.L[i] * (unsigned long long)ci));
+    fwd_fn f = pick_inv(p.L[i], ci);
     if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
     ProfScope ps(i == 0 ? BCG_K_INV_UNPACK : BCG_K_INV, s);
     if (f(i == 0, b.ntiles, 1, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
@@ -1002,7 +1013,9 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
     b.data = da;
     b.coef = (long long *)db;
     b.fast_round = fast;
-    fwd_fn f = pick_inv(p.L[i], p.C[i]);
+    const int ci = inv_cols(p, i);
+    b.ntiles = (int)(p.N / ((unsigned long long)p.L[i] * (unsigned long long)ci));
+    fwd_fn f = pick_inv(p.L[i], ci);
     if (!f) return fail(BCG_E_ARG, "no inverse kernel for this tile");
     ProfScope ps(i == 0 ? BCG_K_INV_UNPACK : BCG_K_INV, s);
     if (f(i == 0, b.ntiles, count, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");

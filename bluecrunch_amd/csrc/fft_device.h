@@ -201,10 +201,13 @@ struct Walk16 {
   }
   // Same, when lanes l, l + C, l + 2C, l + 3C of each 4C-lane block share du
   // (same column): each computes one of g, g^2, g^4, g^8 and the four are
-  // exchanged by lane shuffles. Needs 4C <= 64 and all 4C lanes active.
+  // exchanged by lane shuffles (4C <= 64, all 4C lanes active; else init).
   template <int C>
   __device__ __forceinline__ void init_shared(const Roots &r, unsigned long long u0, unsigned long long du) {
-    static_assert(4 * C <= 64, "four lanes per column within a wave");
+    if constexpr (4 * C > 64) {  // wider tiles: every lane computes its own
+      init(r, u0, du);
+      return;
+    }
     const int lane = threadIdx.x & 63;
     const int q = (lane / C) & 3;
     const int src0 = (lane % C) + (lane / (4 * C)) * (4 * C);

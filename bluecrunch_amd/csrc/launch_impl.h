@@ -38,7 +38,8 @@ int launch_inv(bool unpack, int grid, int gy, hipStream_t s, const PassArgs &a) 
   return 0;
 }
 
-// Column counts with L * C <= 8192 elements per tile.
+// Column counts with L * C <= 8192 elements per tile (32 columns only for the
+// middle inverse pass of 256-point tiles, see inv_cols).
 template <int L, template <int, int> class F>
 struct ByC {
   static fwd_fn pick(int C) {
@@ -48,6 +49,7 @@ struct ByC {
       case 4: return (L * 4 <= 8192) ? &F<L, (L * 4 <= 8192 ? 4 : 1)>::run : nullptr;
       case 8: return (L * 8 <= 8192) ? &F<L, (L * 8 <= 8192 ? 8 : 1)>::run : nullptr;
       case 16: return (L * 16 <= 8192) ? &F<L, (L * 16 <= 8192 ? 16 : 1)>::run : nullptr;
+      case 32: return (L == 256) ? &F<L, (L == 256 ? 32 : 1)>::run : nullptr;
     }
     return nullptr;
   }

@@ -686,7 +686,16 @@ int carry_of_plan(const Plan &p, uint32_t *out, unsigned long long CL, hipStream
   glds = (glds + glds / 32 + 4) * 4;  // one pad word per 32
   {
     ProfScope ps(BCG_K_CARRY, s);
-    hipLaunchKernelGGL(k_carry_groups, dim3((unsigned)nblk), dim3(GCT), glds, s, c);
+    // groups covering one word: ceil(K / (G b / 32)) when group starts are
+    // word-aligned, else the bound of group_words
+    const int gb = p.C[0] * p.bits;
+    const int gcov = gb % 32 == 0 ? (K + gb / 32 - 1) / (gb / 32) : (32 * K + gb - 1) / gb + 1;
+    if (gcov <= 2)
+      hipLaunchKernelGGL(k_carry_groups<2>, dim3((unsigned)nblk), dim3(GCT), glds, s, c);
+    else if (gcov == 3)
+      hipLaunchKernelGGL(k_carry_groups<3>, dim3((unsigned)nblk), dim3(GCT), glds, s, c);
+    else
+      hipLaunchKernelGGL(k_carry_groups<CARRY_GCOV>, dim3((unsigned)nblk), dim3(GCT), glds, s, c);
     LAUNCH(k_carry_fix, 1, 1024, s, c);
     LAUNCH(k_carry_apply, (unsigned)nblk, 256, s, c);
   }

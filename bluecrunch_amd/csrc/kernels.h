@@ -1054,6 +1054,7 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
 // and the carry-function scan of k_carry_local.
 constexpr int GCT = 256;
 constexpr int GWPT = 16;
+template <int GCOV>  // group sums that can cover one word (<= CARRY_GCOV)
 static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
   extern __shared__ uint32_t gl[];
   __shared__ unsigned wave_fn[GCT / 64];
@@ -1097,9 +1098,9 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
   long long w = wfirst - 1;
   long long g = w < 0 ? 0 : div_small(32 * w + 31, gb, inv_gb);
   if (g >= ng) g = ng - 1;
-  long long Wt[CARRY_GCOV];
+  long long Wt[GCOV];
 #pragma unroll
-  for (int t = 0; t < CARRY_GCOV; t++) Wt[t] = ((g - t) * gb) >> 5;
+  for (int t = 0; t < GCOV; t++) Wt[t] = ((g - t) * gb) >> 5;
   unsigned long long P[GWPT + 1];
 #pragma unroll
   for (int k = 0; k <= GWPT; k++) {
@@ -1107,13 +1108,13 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
     if (k > 0 && g + 1 < ng && (((g + 1) * gb) >> 5) <= w) {
       g++;
 #pragma unroll
-      for (int t = CARRY_GCOV - 1; t > 0; t--) Wt[t] = Wt[t - 1];
+      for (int t = GCOV - 1; t > 0; t--) Wt[t] = Wt[t - 1];
       Wt[0] = (g * gb) >> 5;
     }
     unsigned long long p = 0;
     if (w >= 0 && w < (long long)a.CL) {
 #pragma unroll
-      for (int t = 0; t < CARRY_GCOV; t++) {
+      for (int t = 0; t < GCOV; t++) {
         const long long off = w - Wt[t];
         if (g - t >= g_first && off < a.gK) {
           const unsigned i = (unsigned)((g - t - g_first) * a.gKs + off);

@@ -211,6 +211,10 @@ int init_locked(int device) {
   g.device = device;
   g.ready = true;
   if (int rc = ensure(g.margin, sizeof(unsigned int) * MARGIN_SLOTS * MARGIN_STRIDE)) return rc;
+  // every kernel family's code object now rather than inside a first product
+  // (a failure here only defers the load to the first launch)
+  for (auto f : {preload_mul1, preload_fwd_pow2, preload_fwd_three, preload_inv_pow2, preload_inv_three, preload_mid})
+    if (f()) (void)hipGetLastError();
   return 0;
 }
 

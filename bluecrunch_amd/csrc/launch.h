@@ -21,6 +21,14 @@ fwd_fn pick_inv_three(int L, int C);
 mid_fn pick_mid(int L);
 mul1_fn pick_mul1(int L);
 
+// Code-object preloads, one per launch_*.hip unit (called at initialisation).
+int preload_mul1();
+int preload_fwd_pow2();
+int preload_fwd_three();
+int preload_inv_pow2();
+int preload_inv_three();
+int preload_mid();
+
 inline fwd_fn pick_fwd(int L, int C) { return (L % 3 == 0) ? pick_fwd_three(L, C) : pick_fwd_pow2(L, C); }
 inline fwd_fn pick_inv(int L, int C) { return (L % 3 == 0) ? pick_inv_three(L, C) : pick_inv_pow2(L, C); }
 

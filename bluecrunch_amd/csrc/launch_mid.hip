@@ -51,4 +51,11 @@ mid_fn pick_mid(int L) {
   return nullptr;
 }
 
+// Loads this unit's code object now; the runtime otherwise loads it at the
+// first launch of one of its kernels, inside a caller's first product.
+int preload_mid() {
+  hipFuncAttributes fa;
+  return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&(k_mid<1024, false, false, false>))) == hipSuccess ? 0 : -1;
+}
+
 }  // namespace bcg

@@ -30,4 +30,11 @@ mul1_fn pick_mul1(int L) {
   return nullptr;
 }
 
+// Loads this unit's code object now; the runtime otherwise loads it at the
+// first launch of one of its kernels, inside a caller's first product.
+int preload_mul1() {
+  hipFuncAttributes fa;
+  return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_mul1<1024>)) == hipSuccess ? 0 : -1;
+}
+
 }  // namespace bcg

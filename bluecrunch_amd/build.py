@@ -58,7 +58,7 @@ def build(verbose=True, jobs=None, out=None, build_dir=None, defines=()):
             f.result()
     os.makedirs(os.path.dirname(lib_out), exist_ok=True)
     tmp = lib_out + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, api_o, "-o", tmp, "-lm"])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, api_o, "-o", tmp, "-lm", "-pthread"])
     os.replace(tmp, lib_out)
     return lib_out
 

@@ -12,6 +12,7 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/bluecrunch_gpu.h"
@@ -188,7 +189,17 @@ int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
     if (i >= 1) {  // chunk i - 1 out of its region while chunk i is in flight
       const size_t j = i - 1, n = bytes - j * CH < CH ? bytes - j * CH : CH;
       HIPCHK(hipEventSynchronize(g_ring.ev[j & 1]));
-      memcpy((char *)dst + j * CH, st + (j & 1) * CH, n);
+      // four host threads: the destination's first-touch page faults and the
+      // copy itself spread over cores
+      constexpr int NT = 4;
+      const size_t part = (n / NT + 4095) & ~(size_t)4095;
+      std::thread th[NT - 1];
+      for (int k = 1; k < NT; k++) {
+        const size_t a0 = k * part < n ? k * part : n, a1 = (k + 1) * part < n ? (k + 1) * part : n;
+        th[k - 1] = std::thread([=] { memcpy((char *)dst + j * CH + a0, st + (j & 1) * CH + a0, a1 - a0); });
+      }
+      memcpy((char *)dst + j * CH, st + (j & 1) * CH, part < n ? part : n);
+      for (auto &t : th) t.join();
     }
   }
   return 0;

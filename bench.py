@@ -441,7 +441,7 @@ def main():
             "bits_per_digit": s_info["bits"], "points": s_info["points"]},
         "host_pointer": None if host_ms is None else {
             "ms_per_product": host_ms, "digits_per_s": digits / (host_ms * 1e-3),
-            "note": "bcg_mul_host: operands copied in and the product copied out over PCIe (pageable memory)"},
+            "note": "bcg_mul_host: operands copied in and the product copied out over PCIe (8 MiB chunks through pinned staging; the product into a fresh array each call)"},
         "fft_passes_hbm_frac": (fft_bytes / (sec - carry_ms * 1e-3)) / HBM_PEAK if sec > carry_ms * 1e-3 else None,
         "kernels": kern,
         "roofline": roof,

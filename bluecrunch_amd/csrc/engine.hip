@@ -152,13 +152,57 @@ int ring_take(size_t n, char **out) {
   return 0;
 }
 
+// Host copy of n bytes spread over four threads (page-aligned parts): the
+// first-touch page faults of a fresh destination and the copy itself run on
+// several cores.
+void par_memcpy(void *dst, const void *src, size_t n) {
+  constexpr int NT = 4;
+  if (n < ((size_t)1 << 20)) {
+    memcpy(dst, src, n);
+    return;
+  }
+  const size_t part = (n / NT + 4095) & ~(size_t)4095;
+  std::thread th[NT - 1];
+  for (int k = 1; k < NT; k++) {
+    const size_t a0 = k * part < n ? k * part : n, a1 = (k + 1) * part < n ? (k + 1) * part : n;
+    th[k - 1] = std::thread([=] { memcpy((char *)dst + a0, (const char *)src + a0, a1 - a0); });
+  }
+  memcpy(dst, src, part < n ? part : n);
+  for (auto &t : th) t.join();
+}
+
+constexpr size_t RING_CHUNK = (size_t)8 << 20;
+
+int ring_events() {
+  for (int i = 0; i < 2; i++)
+    if (!g_ring.ev[i]) HIPCHK(hipEventCreateWithFlags(&g_ring.ev[i], hipEventDisableTiming));
+  return 0;
+}
+
 // Host to device, queued on s (the host data is copied before returning).
+// Large copies go in 8 MiB chunks through two ring regions: chunk i is copied
+// into region i & 1 once the DMA of chunk i - 2 has left it, so the host copy
+// of one chunk overlaps the DMA of the one before.
 int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
   if (!bytes) return 0;
+  constexpr size_t CH = RING_CHUNK;
   char *st;
-  if (int rc = ring_take(bytes, &st)) return rc;
-  memcpy(st, src, bytes);
-  HIPCHK(hipMemcpyAsync(dst, st, bytes, hipMemcpyHostToDevice, s));
+  if (bytes <= CH) {
+    if (int rc = ring_take(bytes, &st)) return rc;
+    memcpy(st, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, st, bytes, hipMemcpyHostToDevice, s));
+    return 0;
+  }
+  if (int rc = ring_events()) return rc;
+  if (int rc = ring_take(2 * CH, &st)) return rc;
+  const size_t nch = (bytes + CH - 1) / CH;
+  for (size_t i = 0; i < nch; i++) {
+    const size_t n = bytes - i * CH < CH ? bytes - i * CH : CH;
+    if (i >= 2) HIPCHK(hipEventSynchronize(g_ring.ev[i & 1]));
+    par_memcpy(st + (i & 1) * CH, (const char *)src + i * CH, n);
+    HIPCHK(hipMemcpyAsync((char *)dst + i * CH, st + (i & 1) * CH, n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(g_ring.ev[i & 1], s));
+  }
   return 0;
 }
 
@@ -166,7 +210,7 @@ int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
 // Large copies go in 8 MiB chunks through two ring regions, the host copy of
 // one chunk overlapping the DMA of the next.
 int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
-  constexpr size_t CH = (size_t)8 << 20;
+  constexpr size_t CH = RING_CHUNK;
   if (bytes <= CH) {
     char *st;
     if (int rc = ring_take(bytes ? bytes : 1, &st)) return rc;
@@ -175,8 +219,7 @@ int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
     memcpy(dst, st, bytes);
     return 0;
   }
-  for (int i = 0; i < 2; i++)
-    if (!g_ring.ev[i]) HIPCHK(hipEventCreateWithFlags(&g_ring.ev[i], hipEventDisableTiming));
+  if (int rc = ring_events()) return rc;
   char *st;
   if (int rc = ring_take(2 * CH, &st)) return rc;
   const size_t nch = (bytes + CH - 1) / CH;
@@ -189,17 +232,7 @@ int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
     if (i >= 1) {  // chunk i - 1 out of its region while chunk i is in flight
       const size_t j = i - 1, n = bytes - j * CH < CH ? bytes - j * CH : CH;
       HIPCHK(hipEventSynchronize(g_ring.ev[j & 1]));
-      // four host threads: the destination's first-touch page faults and the
-      // copy itself spread over cores
-      constexpr int NT = 4;
-      const size_t part = (n / NT + 4095) & ~(size_t)4095;
-      std::thread th[NT - 1];
-      for (int k = 1; k < NT; k++) {
-        const size_t a0 = k * part < n ? k * part : n, a1 = (k + 1) * part < n ? (k + 1) * part : n;
-        th[k - 1] = std::thread([=] { memcpy((char *)dst + j * CH + a0, st + (j & 1) * CH + a0, a1 - a0); });
-      }
-      memcpy((char *)dst + j * CH, st + (j & 1) * CH, part < n ? part : n);
-      for (auto &t : th) t.join();
+      par_memcpy((char *)dst + j * CH, st + (j & 1) * CH, n);
     }
   }
   return 0;
@@ -1585,12 +1618,13 @@ int bcg_mul_host(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uin
   uint32_t *dA = (uint32_t *)g.wA.p;
   uint32_t *dB = square ? dA : (uint32_t *)g.wB.p;
   uint32_t *dC = (uint32_t *)g.wC.p;
-  if (AL) HIPCHK(hipMemcpyAsync(dA, A, AL * 4, hipMemcpyHostToDevice, s));
-  if (!square && BL) HIPCHK(hipMemcpyAsync(dB, B, BL * 4, hipMemcpyHostToDevice, s));
+  // through the pinned staging ring: C may alias A or B, and the operands are
+  // in staging before the product is written back
+  if (int rc = h2d(dA, A, AL * 4, s)) return rc;
+  if (!square)
+    if (int rc = h2d(dB, B, BL * 4, s)) return rc;
   if (int rc = mul_dev_locked(dC, CL, dA, AL, dB, BL, 0, s, info)) return rc;
-  if (CL) HIPCHK(hipMemcpyAsync(C, dC, CL * 4, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return 0;
+  return d2h(C, dC, CL * 4, s);
 }
 
 int bcg_basecase_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,

@@ -65,6 +65,19 @@ def test_fft_mul_entry_point(bc):
     assert np.array_equal(c, exact_product(a, b)) and 4 <= bits <= 20
 
 
+@pytest.mark.parametrize("al,bl", [(3000001, 2500003), (2097152, 2097153), (4194303, 1)])
+def test_fft_mul_host_in_place_across_staging_chunks(bc, oracle, al, bl):
+    # operands and product larger than one 8 MiB staging chunk, lengths not a
+    # multiple of it; CT is AT's buffer (in-place, as bigfloat_mul calls it)
+    a, b = operands(11, al, bl)
+    want, _, _ = oracle.fft_mul(a, b)
+    buf = np.zeros(al + bl, dtype=np.uint32)
+    buf[:al] = a
+    bits = bc.lib()._fft_mul(bc._u32(buf), al + bl, bc._u32(buf), al, bc._u32(b), bl)
+    assert 4 <= bits <= 20
+    assert np.array_equal(buf, want)
+
+
 def test_basecase_and_karatzuba_entry_points(bc):
     import ctypes
     for al, bl in [(1, 1), (3, 17), (49, 50), (200, 300)]:

@@ -165,10 +165,16 @@ void par_memcpy(void *dst, const void *src, size_t n) {
   std::thread th[NT - 1];
   for (int k = 1; k < NT; k++) {
     const size_t a0 = k * part < n ? k * part : n, a1 = (k + 1) * part < n ? (k + 1) * part : n;
-    th[k - 1] = std::thread([=] { memcpy((char *)dst + a0, (const char *)src + a0, a1 - a0); });
+    auto piece = [=] { memcpy((char *)dst + a0, (const char *)src + a0, a1 - a0); };
+    try {
+      th[k - 1] = std::thread(piece);
+    } catch (...) {  // no thread to be had: this part on the calling thread
+      piece();
+    }
   }
   memcpy(dst, src, part < n ? part : n);
-  for (auto &t : th) t.join();
+  for (auto &t : th)
+    if (t.joinable()) t.join();
 }
 
 constexpr size_t RING_CHUNK = (size_t)8 << 20;

@@ -204,10 +204,12 @@ def batch_leg(bc, count, seed):
             "batched_equals_per_call": same, "graph_equals_batched": same_graph, "exact_spot_checks": exact}
 
 
-def cpu_baseline(words, seed, one_core_words):
+def cpu_baseline(words, seed, one_core_words, gpu_product=None):
     """Oracle _fft_mul (the CPU restatement of the reference AVX path, OpenMP) on
     the host cores: one warm-up then the median of 3 runs of a words x words
-    product on all cores, plus one run of a smaller product on one core."""
+    product on all cores, plus one run of a smaller product on one core.
+    gpu_product (optional): the GPU's product of the same operands, compared
+    word for word with the oracle's."""
     from oracle import refcpu
     from bluecrunch_amd.workload import operands
 
@@ -226,6 +228,8 @@ def cpu_baseline(words, seed, one_core_words):
         if i:
             times.append(time.perf_counter() - t0)
     dt = sorted(times)[1]
+    words_equal = None if gpu_product is None else bool(np.array_equal(c, gpu_product))
+    del c
     one = None
     if one_core_words:
         a1, b1 = operands(seed, one_core_words, one_core_words)
@@ -246,6 +250,12 @@ def cpu_baseline(words, seed, one_core_words):
                   f"median of 3 runs after a warm-up, {dt:.2f} s each on {cores} threads",
         "seconds": dt,
         "one_core": one,
+        # the host the baseline ran on: logical CPUs, CPUs this process may use,
+        # and the OpenMP threads the oracle ran with (= "cores")
+        "host_cpus": os.cpu_count(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
+        "omp_threads": cores,
+        "parity_words_equal": words_equal,
     }
 
 
@@ -409,7 +419,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.cpu_words, args.seed, args.cpu_one_core_words)
+        # dC holds the last timed product of the same seeded operands
+        gpu_c = dC.to_host() if (args.cpu_words == al and not args.no_check) else None
+        cpu = cpu_baseline(args.cpu_words, args.seed, args.cpu_one_core_words, gpu_c)
+        del gpu_c
 
     out = {
         "metric": METRIC,
@@ -450,6 +463,9 @@ def main():
         "batch": b_leg,
         "speedup_vs_cpu_baseline": (value / cpu["value"]) if cpu else None,
         "parity_fingerprint_ok": ok,
+        # the GPU product against the oracle's product, word for word (None
+        # when the CPU leg is skipped or runs another size)
+        "parity_words_equal": None if cpu is None else cpu["parity_words_equal"],
     }
     print(json.dumps(out), flush=True)
 

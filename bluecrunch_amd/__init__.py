@@ -31,18 +31,19 @@ EXPORTS = {
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
-        "bcg_plan_force", "bcg_plan_current", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_prof_enable", "bcg_prof_select", "bcg_prof_read",
+        "bcg_plan_force", "bcg_plan_current", "bcg_plan_digit_bias", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_test_fail_alloc", "bcg_bf_set", "bcg_bf_get", "bcg_bf_free", "bcg_bf_add", "bcg_bf_sub", "bcg_bf_mul", "bcg_bf_rcp", "bcg_bf_div", "bcg_prof_enable", "bcg_prof_select", "bcg_prof_read",
         "bcg_prof_reset", "bcg_stats_reset", "bcg_stats_get",
     ],
     "fft.h": [
         "twiddle_table", "twiddle_table_size", "fft_ensure_table", "table_select", "fft_free_table",
         "fft_length", "fft_forward", "fft_inverse", "fft_pointwise", "int_to_fft", "fft_to_int",
-        "tft_forward", "tft_inverse", "baileys_forward",
+        "tft_forward", "tft_inverse", "baileys_forward", "_baileys_forward",
     ],
     "mul.h": ["_basecase_mul", "_karatzuba_mul", "_fft_mul"],
     "bigfloat.h": [
         "bigfloat_zero", "bigfloat_iszero", "bigfloat_set", "bigfloat_free", "bigfloat_new", "bigfloat_copy",
-        "bigfloat_alloc", "bigfloat_realloc", "bigfloat_mul",
+        "bigfloat_alloc", "bigfloat_realloc", "bigfloat_mul", "bigfloat_mului", "bigfloat_mulu",
+        "bigfloat_add", "bigfloat_sub", "bigfloat_div", "bigfloat_rcp",
     ],
 }
 
@@ -63,6 +64,11 @@ class MulInfo(ctypes.Structure):
 class Bigfloat(ctypes.Structure):
     """include/bigfloat.h bigfloat_t layout."""
     _fields_ = [("coef", _u32p), ("exp", ctypes.c_int64), ("len", _sz), ("sign", ctypes.c_int)]
+
+
+class DevBigfloat(ctypes.Structure):
+    """bcg_bigfloat: the bigfloat_t layout with the words in device memory."""
+    _fields_ = [("coef", _vp), ("exp", ctypes.c_int64), ("len", _sz), ("sign", ctypes.c_int)]
 
 
 class GpuError(RuntimeError):
@@ -116,6 +122,7 @@ def lib():
     L.bcg_plan_query.argtypes = [_sz, _sz, ctypes.POINTER(i), ctypes.POINTER(_sz), ctypes.POINTER(i)]
     L.bcg_plan_candidates.argtypes = [_sz, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
     L.bcg_plan_force.argtypes = [_sz, i]
+    L.bcg_plan_digit_bias.argtypes = [i]
     L.bcg_plan_current.argtypes = [_sz]
     L.bcg_autotune.argtypes = [_sz, _sz, i, i, ctypes.POINTER(i), _f64p, i]
     L.bcg_e_terms.argtypes = [_sz]
@@ -125,6 +132,7 @@ def lib():
     L.bcg_radix_dev.argtypes = [_vp, _sz, _sz, _vp]
     L.bcg_e_digits.argtypes = [_sz, ctypes.c_char_p, _sz, _f64p]
     L.bcg_e_digits_ex.argtypes = [_sz, ctypes.c_char_p, _sz, _f64p, i]
+    L.bcg_test_fail_alloc.argtypes = [ctypes.c_long]
     L.bcg_prof_enable.argtypes = [i]
     L.bcg_prof_select.argtypes = [ctypes.c_uint]
     L.bcg_prof_read.argtypes = [i, ctypes.POINTER(ctypes.c_long), _f64p]
@@ -146,8 +154,17 @@ def lib():
         getattr(L, name).restype = None
     L._fft_mul.argtypes = [_u32p, _sz, _u32p, _sz, _u32p, _sz]
     L._fft_mul.restype = i
-    L.bigfloat_mul.argtypes = [ctypes.POINTER(Bigfloat)] * 3 + [_sz]
-    L.bigfloat_mul.restype = None
+    for name in ("bigfloat_mul", "bigfloat_add", "bigfloat_sub", "bigfloat_div"):
+        getattr(L, name).argtypes = [ctypes.POINTER(Bigfloat)] * 3 + [_sz]
+        getattr(L, name).restype = None
+    L.bigfloat_rcp.argtypes = [ctypes.POINTER(Bigfloat)] * 2 + [_sz]
+    L.bigfloat_rcp.restype = None
+    L.bcg_bf_set.argtypes = [ctypes.POINTER(DevBigfloat), _vp, _sz, ctypes.c_int64, i]
+    L.bcg_bf_get.argtypes = [ctypes.POINTER(DevBigfloat), _vp]
+    L.bcg_bf_free.argtypes = [ctypes.POINTER(DevBigfloat)]
+    for name in ("bcg_bf_add", "bcg_bf_sub", "bcg_bf_mul", "bcg_bf_div"):
+        getattr(L, name).argtypes = [ctypes.POINTER(DevBigfloat)] * 3 + [_sz]
+    L.bcg_bf_rcp.argtypes = [ctypes.POINTER(DevBigfloat)] * 2 + [_sz]
     L.bigfloat_free.argtypes = [ctypes.POINTER(Bigfloat)]
     L.bigfloat_free.restype = None
     _lib = L
@@ -279,9 +296,10 @@ def fft_mul(a, b, cl=None):
     return c[:cl], bits
 
 
-def bigfloat_mul(a, b, p=0, inplace=None):
-    """Reference bigfloat_mul on (words, exp, sign) triples; `a is b` is squaring.
-    inplace="a" makes the target the same bigfloat as a (test/test_regression.c)."""
+def _bigfloat_host_call(fn, a, b, p, inplace=None):
+    """A reference bigfloat operation of the library (host bigfloat_t) on
+    (words, exp, sign) triples; b None for the one-operand rcp, `b is a` passes
+    the same bigfloat twice. inplace="a" makes the target the bigfloat a."""
     L = lib()
     libc = ctypes.CDLL(None)
     libc.malloc.restype = _vp
@@ -301,19 +319,83 @@ def bigfloat_mul(a, b, p=0, inplace=None):
         return bf
 
     A = mk(a)
-    B = A if b is a else mk(b)
+    B = None if b is None else (A if b is a else mk(b))
     if inplace == "a":
         T = A
     else:
         T = Bigfloat()
         T.coef = ctypes.cast(None, _u32p)
         T.len, T.exp, T.sign = 0, 0, 1
-    L.bigfloat_mul(ctypes.byref(T), ctypes.byref(A), ctypes.byref(B), p)
+    if B is None:
+        fn(ctypes.byref(T), ctypes.byref(A), p)
+    else:
+        fn(ctypes.byref(T), ctypes.byref(A), ctypes.byref(B), p)
     words = np.ctypeslib.as_array(T.coef, shape=(T.len,)).copy() if T.len else np.zeros(0, np.uint32)
     res = (words, T.exp, T.sign)
-    for bf in {id(x): x for x in (A, B, T)}.values():
+    for bf in {id(x): x for x in (A, B, T) if x is not None}.values():
         L.bigfloat_free(ctypes.byref(bf))
     return res
+
+
+def bigfloat_mul(a, b, p=0, inplace=None):
+    """Reference bigfloat_mul on (words, exp, sign) triples; `a is b` is squaring.
+    inplace="a" makes the target the same bigfloat as a (test/test_regression.c)."""
+    return _bigfloat_host_call(lib().bigfloat_mul, a, b, p, inplace)
+
+
+def bigfloat_add(a, b, p=0, inplace=None):
+    return _bigfloat_host_call(lib().bigfloat_add, a, b, p, inplace)
+
+
+def bigfloat_sub(a, b, p=0, inplace=None):
+    return _bigfloat_host_call(lib().bigfloat_sub, a, b, p, inplace)
+
+
+def bigfloat_div(a, b, p=0, inplace=None):
+    return _bigfloat_host_call(lib().bigfloat_div, a, b, p, inplace)
+
+
+def bigfloat_rcp(a, p, inplace=None):
+    return _bigfloat_host_call(lib().bigfloat_rcp, a, None, p, inplace)
+
+
+class DeviceBigfloat:
+    """A bcg_bigfloat (device words) made from a (words, exp, sign) triple."""
+
+    def __init__(self, t=None):
+        self.bf = DevBigfloat()
+        self.bf.sign = 1
+        if t is not None:
+            w, e, s = t
+            w = np.ascontiguousarray(w, dtype=np.uint32)
+            check(lib().bcg_bf_set(ctypes.byref(self.bf), w.ctypes.data if w.size else None, w.size, e, s),
+                  "bcg_bf_set")
+
+    def triple(self):
+        n = self.bf.len
+        w = np.zeros(max(n, 1), dtype=np.uint32)
+        check(lib().bcg_bf_get(ctypes.byref(self.bf), w.ctypes.data), "bcg_bf_get")
+        return w[:n], self.bf.exp, self.bf.sign
+
+    def free(self):
+        if lib() is not None and self.bf.coef:
+            lib().bcg_bf_free(ctypes.byref(self.bf))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def bf_op(name, t, a, b=None, p=0):
+    """t = a (op) b on device bigfloats (DeviceBigfloat objects; t may be a or b)."""
+    fn = getattr(lib(), "bcg_bf_" + name)
+    if b is None:
+        check(fn(ctypes.byref(t.bf), ctypes.byref(a.bf), p), "bcg_bf_" + name)
+    else:
+        check(fn(ctypes.byref(t.bf), ctypes.byref(a.bf), ctypes.byref(b.bf), p), "bcg_bf_" + name)
+    return t
 
 
 # ---------------------------------------------------------- device buffers
@@ -334,6 +416,12 @@ class DeviceWords:
         if arr.size:
             check(lib().bcg_memcpy_h2d(d.ptr, arr.ctypes.data, 4 * arr.size), "h2d")
         return d
+
+    def zero(self):
+        """Clear the words (from a host array held for the whole copy)."""
+        z = np.zeros(max(self.n, 1), dtype=np.uint32)
+        if self.n:
+            check(lib().bcg_memcpy_h2d(self.ptr, z.ctypes.data, 4 * self.n), "h2d")
 
     def to_host(self, n=None):
         n = self.n if n is None else n
@@ -381,6 +469,12 @@ def plan_force(n, index):
     check(lib().bcg_plan_force(n, index), "plan_force")
 
 
+def plan_digit_bias(extra_bits):
+    """Plan digit sizes up to extra_bits above the rounding bound (0: normal);
+    exercises the rounding-margin retry."""
+    check(lib().bcg_plan_digit_bias(extra_bits), "plan_digit_bias")
+
+
 def plan_current(n):
     """Index into plan_candidates(n) of the split n-point transforms use now."""
     r = lib().bcg_plan_current(n)
@@ -425,7 +519,7 @@ def prof_reset():
 class Stats(ctypes.Structure):
     """bcg_stats: running statistics over FFT products since the last reset."""
     _fields_ = [("worst_margin", ctypes.c_double), ("products", ctypes.c_size_t), ("max_points", ctypes.c_size_t),
-                ("bits_at_max", ctypes.c_int), ("margin_at_max", ctypes.c_double)]
+                ("bits_at_max", ctypes.c_int), ("margin_at_max", ctypes.c_double), ("retries", ctypes.c_size_t)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -542,7 +636,7 @@ def e_bsr(terms):
 
 
 def div_fraction(p_words, q_words, nw):
-    """floor(P * 2^(32 nw) / Q) (nw words, a few units of error) on the GPU."""
+    """floor(P * 2^(32 nw) / Q) (nw words, exact) on the GPU, for P < Q."""
     dP, dQ = DeviceWords.from_host(p_words), DeviceWords.from_host(q_words)
     f = _vp()
     check(lib().bcg_div_fraction_dev(dP.ptr, len(p_words), dQ.ptr, len(q_words), nw, ctypes.byref(f)),

@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     f"-I{INCLUDE}",
 ]
-C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", f"-I{INCLUDE}"]
+C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra", "-Wno-unused-parameter", f"-I{INCLUDE}"]
 
 
 def _run(cmd):

@@ -1,10 +1,13 @@
 /*
  * api.c -- the reference's C API (include/fft.h, include/mul.h and
  * bigfloat_mul from include/bigfloat.h) on top of the HIP engine's C ABI
- * (include/bluecrunch_gpu.h). Host-side bookkeeping only; every arithmetic
- * step runs on the GPU. Errors follow the reference convention: a message on
- * stderr, then abort().
+ * (include/bluecrunch_gpu.h). Host-side bookkeeping only; every transform,
+ * product and carry runs on the GPU. The one exception is the pair of
+ * word-by-uint32 products bigfloat_mului/bigfloat_mulu, O(len) scalar loops
+ * that the reference also runs on the host and that involve no transform.
+ * Errors follow the reference convention: a message on stderr, then abort().
  */
+#define _GNU_SOURCE /* M_PI */
 #include <complex.h>
 #include <math.h>
 #include <stdio.h>
@@ -48,20 +51,31 @@ size_t table_select(size_t length) {
   return idx;
 }
 
-/* fft/table.c:189-209: L/2 roots e^{+2 pi i c/L} (long double, then rounded). */
+/* fft/table.c:189-209: the L/2 roots e^{+2 pi i c/L}, c < L/2, computed as
+ * the reference computes them (omega = 2 pi / L in double, then libm cos and
+ * sin of omega * c), into slot _table_select(L/2). A slot that already holds a
+ * table keeps it (the reference builds a new one and leaks the old). */
 static void build_one(size_t L) {
   size_t slot = slot_of(L / 2);
   if (slot >= 125 || twiddle_table[slot]) return;
   complex double *t = (complex double *)aligned_alloc(32, sizeof(complex double) * ((L / 2 + 1) & ~(size_t)1));
+  if (!t) {
+    fprintf(stderr, "fft_ensure_table: out of memory\n");
+    abort();
+  }
+  const double omega = 2 * M_PI / (double)L;
   for (size_t c = 0; c < L / 2; c++) {
-    long double a = 6.283185307179586476925286766559005768L * (long double)c / (long double)L;
-    t[c] = (double)cosl(a) + I * (double)sinl(a);
+    const double angle = omega * (double)c;
+    t[c] = cos(angle) + I * sin(angle);
   }
   twiddle_table[slot] = t;
 }
 
-/* fft/table.c:211-228: host tables for every 2^k and 3*2^k up to length, and a
- * GPU warm-up so later calls do not pay for device set-up. */
+/* fft/table.c:211-228: host tables for every 2^k and 3*2^k up to length;
+ * twiddle_table_size becomes the slot count of this call (as in the
+ * reference, also when it is smaller than an earlier call's). The GPU is set
+ * up here too, so that later calls do not pay for device initialisation. The
+ * GPU's own transforms never read these tables. */
 void fft_ensure_table(size_t length) {
   size_t k = 2;
   for (;;) {
@@ -71,12 +85,14 @@ void fft_ensure_table(size_t length) {
     build_one(3 * k);
     k *= 2;
   }
-  int size = (int)slot_of(k) + 1;
-  if (size > twiddle_table_size) twiddle_table_size = size;
+  twiddle_table_size = (int)slot_of(k) + 1;
   GPU(bcg_init(0));
 }
 
-/* fft/table.c:230-234 */
+/* fft/table.c:230-234. Unlike the reference (which frees the first
+ * twiddle_table_size tables and leaves the pointers and the size behind), the
+ * pointers are cleared and the size reset, so table_select reports no table
+ * afterwards instead of a freed one. */
 void fft_free_table() {
   for (int i = 0; i < 125; i++) {
     free(twiddle_table[i]);
@@ -122,6 +138,11 @@ void fft_inverse(complex double *T, size_t length) {
 /* The four-step transform is how the engine runs every transform above 8192
  * points, so the Bailey entry point is the forward transform itself. */
 void baileys_forward(complex double *T, size_t length) { fft_forward(T, length); }
+
+/* fft/fft.c:149-232: the reference's Bailey four-step, whose output order
+ * equals the forward transform's (fft.h does not declare it; its test,
+ * test/test_baileys.c:17, declares it itself). */
+void _baileys_forward(complex double *T, size_t length) { fft_forward(T, length); }
 
 /* fft/pointwise.c:9-20 */
 void fft_pointwise(complex double *a, complex double *b, size_t len) {
@@ -239,6 +260,35 @@ void bigfloat_realloc(bigfloat_t target, size_t size) {
   if (size > old) memset(target->coef + old, 0, sizeof(uint32_t) * (size - old));
 }
 
+/* ------------------------------------------- products by one 32-bit word */
+
+/* bigfloat/mul.c:64-81: a *= b in place over a->len words. The caller keeps a
+ * spare top word for the product (the reference clears it first, then the
+ * final carry out of the top word is dropped). O(len) scalar work on host
+ * words, the same as the reference's: no transform is involved. */
+void bigfloat_mului(bigfloat_t a, uint32_t b) {
+  a->coef[a->len - 1] = 0;
+  uint64_t carry = 0;
+  for (size_t i = 0; i < a->len; i++) {
+    const uint64_t v = (uint64_t)a->coef[i] * b + carry;
+    a->coef[i] = (uint32_t)v;
+    carry = v >> 32;
+  }
+}
+
+/* bigfloat/mul.c:83-102: target's first target->len words = a * b (words of a
+ * past its length read as zero; the carry out of the last word is dropped).
+ * target's length, exponent and sign are left as they are. */
+void bigfloat_mulu(bigfloat_t target, const bigfloat_t a, uint32_t b) {
+  uint64_t carry = 0;
+  for (size_t i = 0; i < target->len; i++) {
+    uint64_t v = carry;
+    if (i < a->len) v += (uint64_t)a->coef[i] * b;
+    target->coef[i] = (uint32_t)v;
+    carry = v >> 32;
+  }
+}
+
 /* ---------------------------------------------------------- bigfloat_mul */
 
 /* bigfloat/mul.c:555-676: zero check, precision p (default AL+BL, else
@@ -302,4 +352,78 @@ void bigfloat_mul(bigfloat_t target, const bigfloat_t a, const bigfloat_t b, siz
   if (target->coef[target->len - 1] == 0) target->len--;
   if (target->len > 1000)
     while (target->coef[target->len - 1] == 0) target->len--;
+}
+
+/* ------------------------------------------- add, sub, reciprocal, quotient */
+
+/* The operands go to the GPU as device bigfloats, the operation runs there
+ * with the reference's semantics (include/bluecrunch_gpu.h bcg_bf_*), and the
+ * result replaces target's mantissa. */
+static void bf_up(bcg_bigfloat *d, const bigfloat_t x) {
+  memset(d, 0, sizeof *d);
+  d->sign = 1;
+  GPU(bcg_bf_set(d, x->coef, x->len, x->exp, x->sign));
+}
+
+static void bf_down(bigfloat_t target, bcg_bigfloat *d) {
+  uint32_t *w = NULL;
+  if (d->len) {
+    w = (uint32_t *)malloc(sizeof(uint32_t) * d->len);
+    if (!w) {
+      fprintf(stderr, "bigfloat: out of memory\n");
+      abort();
+    }
+    GPU(bcg_bf_get(d, w));
+  }
+  free(target->coef);
+  target->coef = w;
+  target->len = d->len;
+  target->exp = d->exp;
+  target->sign = d->sign;
+  GPU(bcg_bf_free(d));
+}
+
+typedef int (*bf_op2)(bcg_bigfloat *, const bcg_bigfloat *, const bcg_bigfloat *, size_t);
+
+static void bf_binary(bf_op2 op, bigfloat_t target, const bigfloat_t a, const bigfloat_t b, size_t p) {
+  bcg_bigfloat A, B, T;
+  memset(&T, 0, sizeof T);
+  T.sign = 1;
+  bf_up(&A, a);
+  if (b != a) bf_up(&B, b);
+  GPU(op(&T, &A, b != a ? &B : &A, p));
+  GPU(bcg_bf_free(&A));
+  if (b != a) GPU(bcg_bf_free(&B));
+  bf_down(target, &T);
+}
+
+/* bigfloat/addsub.c:173-198 */
+void bigfloat_add(bigfloat_t target, const bigfloat_t a, const bigfloat_t b, size_t p) {
+  bf_binary(bcg_bf_add, target, a, b, p);
+}
+
+/* bigfloat/addsub.c:200-225 */
+void bigfloat_sub(bigfloat_t target, const bigfloat_t a, const bigfloat_t b, size_t p) {
+  bf_binary(bcg_bf_sub, target, a, b, p);
+}
+
+/* bigfloat/div.c:14-27 */
+void bigfloat_div(bigfloat_t target, const bigfloat_t a, const bigfloat_t b, size_t p) {
+  bf_binary(bcg_bf_div, target, a, b, p);
+}
+
+/* bigfloat/rcp.c:15-101 (a zero operand: "Divide by Zero" and abort, as the
+ * reference) */
+void bigfloat_rcp(bigfloat_t target, const bigfloat_t a, size_t p) {
+  if (a->len == 0) {
+    fprintf(stderr, "Divide by Zero\n");
+    abort();
+  }
+  bcg_bigfloat A, T;
+  memset(&T, 0, sizeof T);
+  T.sign = 1;
+  bf_up(&A, a);
+  GPU(bcg_bf_rcp(&T, &A, p));
+  GPU(bcg_bf_free(&A));
+  bf_down(target, &T);
 }

@@ -646,6 +646,10 @@ bool factor_cached(unsigned long long N, Plan &p) {
   return true;
 }
 
+// Digit sizes planned above the rounding bound b_max(N) (bcg_plan_digit_bias;
+// 0 in normal use).
+int g_digit_bias = 0;
+
 int make_plan(unsigned long long AL, unsigned long long BL, int force_bits, Plan &best) {
   best = Plan();
   unsigned long long bestN = 0;
@@ -658,7 +662,7 @@ int make_plan(unsigned long long AL, unsigned long long BL, int force_bits, Plan
     unsigned long long need = dA + dB - 1;
     unsigned long long N = class_ceil((need + 1) / 2);
     if (bestN && N > bestN) break;
-    if (force_bits <= 0 && b > bmax_for(N)) continue;
+    if (force_bits <= 0 && b > bmax_for(N) + g_digit_bias) continue;
     if (bestN == 0 || N < bestN || (N == bestN && b < bestb)) {
       bestN = N;
       bestb = b;
@@ -955,16 +959,25 @@ int basecase_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uin
                  hipStream_t s, const uint32_t *D = nullptr, size_t DL = 0) {
   if (AL + BL > 4000) return fail(BCG_E_ARG, "basecase limited to AL + BL <= 4000 words");
   size_t sh = (AL + BL + 1) * 2 * sizeof(unsigned long long);
-  // write into a scratch so C may alias A or B
-  if (int rc = ensure(g.zbuf, CL * sizeof(uint32_t) + 64)) return rc;
+  // an output overlapping an input (in-place products) goes through a scratch;
+  // otherwise the kernel writes C directly
+  const uintptr_t c0 = (uintptr_t)C, c1 = c0 + CL * sizeof(uint32_t);
+  auto overlaps = [&](const uint32_t *x, size_t n) {
+    return x && n && (uintptr_t)x < c1 && c0 < (uintptr_t)(x + n);
+  };
+  const bool via_scratch = overlaps(A, AL) || overlaps(B, BL) || overlaps(D, DL);
+  uint32_t *dst = C;
+  if (via_scratch) {
+    if (int rc = ensure(g.zbuf, CL * sizeof(uint32_t) + 64)) return rc;
+    dst = (uint32_t *)g.zbuf.p;
+  }
   {
     ProfScope ps(BCG_K_OTHER, s);
     hipLaunchKernelGGL(k_basecase, dim3(1), dim3(256), sh, s, A, (unsigned long long)AL, B,
-                       (unsigned long long)BL, (uint32_t *)g.zbuf.p, (unsigned long long)CL, D,
-                       (unsigned long long)DL);
+                       (unsigned long long)BL, dst, (unsigned long long)CL, D, (unsigned long long)DL);
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(C, g.zbuf.p, CL * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  if (via_scratch) HIPCHK(hipMemcpyAsync(C, dst, CL * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   return 0;
 }
 
@@ -1203,6 +1216,7 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
         if (int rc = make_plan(AL, BL, p.bits - 1, q)) return rc;
         p = q;
         inf.retries++;
+        g_stats.retries++;
         continue;
       }
       if (m >= 0.25) return fail(BCG_E_MARGIN, "rounding margin >= 0.25 at the smallest digit size");
@@ -1234,6 +1248,7 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
       if (int rc = make_plan(AL, BL, p.bits - 1, q)) return rc;
       p = q;
       inf.retries++;
+      g_stats.retries++;
       continue;
     }
     if (alias) {
@@ -1485,6 +1500,7 @@ __global__ void k_fill_hash(uint32_t *p, unsigned long long n, unsigned long lon
 }
 
 #include "e_pipeline.inc"
+#include "bigfloat_dev.inc"
 
 }  // namespace
 
@@ -1885,6 +1901,7 @@ int batch_finish(const BatchPlan &bp, BatchBufs &bb, hipStream_t s, double *wors
   for (size_t i : retry) {
     int fb = bp.plans[i].bits - 1;
     if (fb < 4) return fail(BCG_E_MARGIN, "rounding margin >= 0.25 at the smallest digit size");
+    g_stats.retries++;
     if (int rc = one(i, fb)) return rc;
   }
   HIPCHK(hipStreamSynchronize(s));
@@ -2187,6 +2204,13 @@ int bcg_plan_candidates(size_t N, int index, int *passes, int *L, int *C) {
   return (int)c.size();
 }
 
+int bcg_plan_digit_bias(int extra_bits) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  if (extra_bits < 0 || extra_bits > 8) return fail(BCG_E_ARG, "digit bias must be in [0, 8]");
+  g_digit_bias = extra_bits;
+  return 0;
+}
+
 int bcg_plan_force(size_t N, int index) {
   std::lock_guard<std::recursive_mutex> lk(g_lock);
   if (index < 0) {
@@ -2283,6 +2307,7 @@ int bcg_div_fraction_dev(const uint32_t *P, size_t pl, const uint32_t *Q, size_t
   if (!P || !Q || !pl || !ql || !nw || !F) return fail(BCG_E_ARG, "bad arguments");
   DInt f;
   int rc = d_divide_fraction(P, pl, Q, ql, nw, f, g.stream);
+  if (!rc) rc = d_floor_fix(P, pl, Q, ql, nw, f, g.stream);
   if (!rc && hipStreamSynchronize(g.stream) != hipSuccess) rc = fail(BCG_E_HIP, "division");
   if (!rc) {
     ep_detach(f.p);
@@ -2318,6 +2343,106 @@ int bcg_radix_dev(const uint32_t *F, size_t nf, size_t groups, uint32_t *digits)
   const int rc = radix_convert(F, nf, groups, digits, g.stream);
   ep_trim(EP_KEEP);
   return rc;
+}
+
+// Device bigfloats (bigfloat_dev.inc): operands copied so that the target may
+// be one of them; the target's old mantissa is released by the operation.
+int bcg_bf_set(bcg_bigfloat *x, const uint32_t *words, size_t len, int64_t exp, int sign) {
+  ENTER();
+  if (!x || (len && !words)) return fail(BCG_E_ARG, "null pointer");
+  uint32_t *w = nullptr;
+  if (len) {
+    hipError_t e = ep_malloc(&w, len * sizeof(uint32_t));
+    if (e != hipSuccess) return fail(BCG_E_NOMEM, "bigfloat words", e);
+    if (int rc = h2d(w, words, len * sizeof(uint32_t), g.stream)) {
+      bf_release(w);
+      return rc;
+    }
+  }
+  bf_release(x->coef);
+  x->coef = w;
+  x->len = len;
+  x->exp = len ? exp : 0;
+  x->sign = len ? (sign ? 1 : 0) : 1;
+  return 0;
+}
+
+int bcg_bf_get(const bcg_bigfloat *x, uint32_t *words) {
+  ENTER();
+  if (!x || (x->len && (!words || !x->coef))) return fail(BCG_E_ARG, "null pointer");
+  if (!x->len) return 0;
+  return d2h(words, x->coef, x->len * sizeof(uint32_t), g.stream);
+}
+
+int bcg_bf_free(bcg_bigfloat *x) {
+  ENTER();
+  if (!x) return 0;
+  bf_release(x->coef);
+  bf_make_zero(*x);
+  ep_trim(EP_KEEP);
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+template <class F>
+int bf_call(bcg_bigfloat *t, const bcg_bigfloat *a, const bcg_bigfloat *b, F op) {
+  if (!t || !a || (a->len && !a->coef) || (b && b->len && !b->coef)) return fail(BCG_E_ARG, "null pointer");
+  bcg_bigfloat T = *t, A = *a, B = b ? *b : bcg_bigfloat();
+  const int rc = op(T, A, B);
+  *t = T;
+  ep_trim(EP_KEEP);
+  return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int bcg_bf_add(bcg_bigfloat *t, const bcg_bigfloat *a, const bcg_bigfloat *b, size_t p) {
+  ENTER();
+  if (!b) return fail(BCG_E_ARG, "null pointer");
+  return bf_call(t, a, b, [&](bcg_bigfloat &T, bcg_bigfloat &A, bcg_bigfloat &B) {
+    return bf_add(T, A, B, p, g.stream);
+  });
+}
+
+int bcg_bf_sub(bcg_bigfloat *t, const bcg_bigfloat *a, const bcg_bigfloat *b, size_t p) {
+  ENTER();
+  if (!b) return fail(BCG_E_ARG, "null pointer");
+  return bf_call(t, a, b, [&](bcg_bigfloat &T, bcg_bigfloat &A, bcg_bigfloat &B) {
+    return bf_sub(T, A, B, p, g.stream);
+  });
+}
+
+int bcg_bf_mul(bcg_bigfloat *t, const bcg_bigfloat *a, const bcg_bigfloat *b, size_t p) {
+  ENTER();
+  if (!b) return fail(BCG_E_ARG, "null pointer");
+  return bf_call(t, a, b, [&](bcg_bigfloat &T, bcg_bigfloat &A, bcg_bigfloat &B) {
+    return bf_mul(T, A, B, p, g.stream);
+  });
+}
+
+int bcg_bf_rcp(bcg_bigfloat *t, const bcg_bigfloat *a, size_t p) {
+  ENTER();
+  return bf_call(t, a, nullptr, [&](bcg_bigfloat &T, bcg_bigfloat &A, bcg_bigfloat &) {
+    return bf_rcp(T, A, p, g.stream);
+  });
+}
+
+int bcg_bf_div(bcg_bigfloat *t, const bcg_bigfloat *a, const bcg_bigfloat *b, size_t p) {
+  ENTER();
+  if (!b) return fail(BCG_E_ARG, "null pointer");
+  return bf_call(t, a, b, [&](bcg_bigfloat &T, bcg_bigfloat &A, bcg_bigfloat &B) {
+    return bf_div(T, A, B, p, g.stream);
+  });
+}
+
+int bcg_test_fail_alloc(long nth) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  if (nth < 0) return fail(BCG_E_ARG, "nth must be >= 0");
+  g_ep_fail_at = nth;
+  return 0;
 }
 
 int bcg_prof_enable(int on) {

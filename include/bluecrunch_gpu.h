@@ -140,6 +140,12 @@ int bcg_plan_candidates(size_t N, int index, int *passes, int *L, int *C);
 /* Use candidate `index` for every transform of N points (-1: back to the
  * default or the autotuned choice). */
 int bcg_plan_force(size_t N, int index);
+/* Plan digit sizes up to extra_bits (0..8) above the rounding bound b_max(N)
+ * of every later product (0, the default: at most b_max). No counterpart in
+ * the reference; it exists to exercise the rounding-margin safety net: with a
+ * bias the first attempt of a product tends to reach a margin >= 0.25 and is
+ * re-run with smaller digits, which bcg_stats counts as retries. */
+int bcg_plan_digit_bias(int extra_bits);
 /* Index of the candidate transforms of N points use now (forced, autotuned
  * or default); negative BCG_E* code when N has no split. */
 int bcg_plan_current(size_t N);
@@ -155,10 +161,11 @@ int bcg_autotune(size_t AL, size_t BL, int square, int reps, int *chosen, double
  * *pq receives one device allocation holding P (pl words) then Q (ql words),
  * released with bcg_free. */
 size_t bcg_e_terms(size_t words);
-/* F = floor(P 2^{32 nw} / Q) (nw words, to within a few units in the last
- * word) for P < Q, by a Newton reciprocal of Q on the GPU (the role of
- * bigfloat_rcp/bigfloat_div, bigfloat/rcp.c:15-101, div.c:14-27). *F is a
- * device allocation released with bcg_free. */
+/* F = floor(P 2^{32 nw} / Q) exactly (nw words) for P < Q (top word of Q
+ * nonzero), by a Newton reciprocal of Q on the GPU and a remainder correction
+ * (the e pipeline's division; bigfloat_rcp/bigfloat_div below keep the
+ * reference's own truncations). *F is a device allocation released with
+ * bcg_free. */
 int bcg_div_fraction_dev(const uint32_t *P, size_t pl, const uint32_t *Q, size_t ql, size_t nw, uint32_t **F);
 /* Base-1e9 digits (most significant first) of the binary fraction F (nf
  * device words, value F / 2^{32 nf}), `groups` of them into device `digits`
@@ -175,6 +182,40 @@ int bcg_e_digits(size_t digits, char *out, size_t cap, double *ms);
 #define BCG_E_PER_MULTIPLY 1
 int bcg_e_digits_ex(size_t digits, char *out, size_t cap, double *ms, int flags);
 int bcg_e_bsr_dev(size_t terms, uint32_t **pq, size_t *pl, size_t *ql);
+
+/* Bigfloats whose words live in device memory: the layout of the reference's
+ * bigfloat_t (include/bigfloat.h:6-13) with coef a device pointer. value =
+ * (sign ? +1 : -1) * sum coef[i] 2^{32 (i + exp)}; len == 0 is zero. The
+ * operations below have the reference's semantics and give its exact words,
+ * exponent and sign (truncation to p + 2 words, p = 0 full precision); the
+ * target may be one of the operands, and its old mantissa is released.
+ * Mantissas come from bcg_bf_set or an operation and go back with bcg_bf_free;
+ * a zero-initialised bcg_bigfloat is a valid zero target. */
+typedef struct {
+  uint32_t *coef; /* device words */
+  int64_t exp;
+  size_t len;
+  int sign; /* 1: positive, 0: negative */
+} bcg_bigfloat;
+/* x = host words (len of them), exponent and sign; x's old mantissa released. */
+int bcg_bf_set(bcg_bigfloat *x, const uint32_t *words, size_t len, int64_t exp, int sign);
+/* x->len words of x to host memory. */
+int bcg_bf_get(const bcg_bigfloat *x, uint32_t *words);
+/* Release x's mantissa; x becomes zero. */
+int bcg_bf_free(bcg_bigfloat *x);
+/* t = a + b, a - b (bigfloat/addsub.c:173-225), a * b (bigfloat/mul.c:555-676),
+ * 1 / a (bigfloat/rcp.c:15-101), a / b (bigfloat/div.c:14-27) at precision p. */
+int bcg_bf_add(bcg_bigfloat *t, const bcg_bigfloat *a, const bcg_bigfloat *b, size_t p);
+int bcg_bf_sub(bcg_bigfloat *t, const bcg_bigfloat *a, const bcg_bigfloat *b, size_t p);
+int bcg_bf_mul(bcg_bigfloat *t, const bcg_bigfloat *a, const bcg_bigfloat *b, size_t p);
+int bcg_bf_rcp(bcg_bigfloat *t, const bcg_bigfloat *a, size_t p);
+int bcg_bf_div(bcg_bigfloat *t, const bcg_bigfloat *a, const bcg_bigfloat *b, size_t p);
+
+/* Test hook for the error paths of the e pipeline (binary splitting,
+ * division, radix conversion): the nth device allocation those make from now
+ * on fails as if the device were out of memory (0: off; the hook clears itself
+ * when it fires). No counterpart in the reference. */
+int bcg_test_fail_alloc(long nth);
 
 /* Kernel timing with HIP events recorded on the launch stream. While enabled,
  * every kernel the engine launches is bracketed by events; bcg_prof_read
@@ -205,6 +246,7 @@ typedef struct {
   size_t max_points;        /* longest transform, complex points */
   int bits_at_max;          /* digit size at that length */
   double margin_at_max;     /* worst margin at that length */
+  size_t retries;           /* re-runs with a smaller digit size (margin >= 0.25) */
 } bcg_stats;
 int bcg_stats_reset(void);
 int bcg_stats_get(bcg_stats *out);

@@ -50,4 +50,8 @@ void tft_inverse(complex double *, size_t, int);
 /* fft.h:31 -- four-step transform; same result and order as fft_forward */
 void baileys_forward(complex double *, size_t);
 
+/* fft/fft.c:149-232 -- not in the reference fft.h (test/test_baileys.c:17
+ * declares it); the same transform and order as fft_forward. */
+void _baileys_forward(complex double *, size_t);
+
 #endif

@@ -526,3 +526,204 @@ void ref_set_threads(int n) {
   (void)n;
 #endif
 }
+
+/* ------------------------------------------------- add, sub, rcp, div */
+
+static void ref_zero(ref_bigfloat *t) { /* bigfloat/zero.c */
+  free(t->coef);
+  t->coef = NULL;
+  t->len = 0;
+  t->exp = 0;
+  t->sign = 1;
+}
+
+/* bigfloat/addsub.c:17-28: the word of x at magnitude mag (0 outside). */
+static uint32_t word_at(const ref_bigfloat *x, int64_t mag) {
+  if (mag < x->exp || mag >= x->exp + (int64_t)x->len) return 0;
+  return x->coef[mag - x->exp];
+}
+
+/* bigfloat/addsub.c:34-64: sign of |a| - |b|, from the top magnitudes down. */
+int ref_bigfloat_ucmp(const ref_bigfloat *a, const ref_bigfloat *b) {
+  const int64_t am = a->exp + (int64_t)a->len, bm = b->exp + (int64_t)b->len;
+  if (am != bm) return am > bm ? 1 : -1;
+  for (int64_t m = am; m >= a->exp || m >= b->exp; m--) {
+    const uint32_t x = word_at(a, m), y = word_at(b, m);
+    if (x != y) return x > y ? 1 : -1;
+  }
+  return 0;
+}
+
+/* The window [bot, top) of both operands, truncated to p (+2) words from the
+ * top as in bigfloat/addsub.c:66-88 (and :119-140); *exp gets the reference's
+ * exponent top - p, which is not bot when the untruncated window is shorter
+ * than p (a quirk kept as it is). */
+static int64_t add_window(const ref_bigfloat *a, const ref_bigfloat *b, size_t p, int64_t *bot, int64_t *exp) {
+  const int64_t am = a->exp + (int64_t)a->len, bm = b->exp + (int64_t)b->len;
+  const int64_t top = am > bm ? am : bm;
+  int64_t lo = a->exp < b->exp ? a->exp : b->exp;
+  int64_t TL = top - lo;
+  if (p == 0)
+    p = (size_t)TL;
+  else
+    p += 2; /* BIGFLOAT_EXTRA_PRECISION */
+  if (TL > (int64_t)p) {
+    lo = top - (int64_t)p;
+    TL = (int64_t)p;
+  }
+  *bot = lo;
+  *exp = top - (int64_t)p;
+  return TL;
+}
+
+/* Replace t's mantissa by w (n words). The reference writes in place when the
+ * target shares an operand's mantissa (addsub.c:90-98); here every operand
+ * word is read before t changes, which equals the reference wherever its
+ * in-place loop reads each word before overwriting it. */
+static void set_mantissa(ref_bigfloat *t, uint32_t *w) {
+  free(t->coef);
+  t->coef = w;
+}
+
+/* bigfloat/addsub.c:66-117: |a| + |b| in the window, sign of a, carry out
+ * appended as a top word. */
+static void ref_uadd(ref_bigfloat *t, const ref_bigfloat *a, const ref_bigfloat *b, size_t p) {
+  int64_t bot, exp;
+  const int64_t TL = add_window(a, b, p, &bot, &exp);
+  uint32_t *w = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(TL + 1));
+  uint64_t carry = 0;
+  for (int64_t c = 0; c < TL; c++) {
+    const uint64_t v = (uint64_t)word_at(a, bot + c) + word_at(b, bot + c) + carry;
+    w[c] = (uint32_t)v;
+    carry = v >> 32;
+  }
+  size_t len = (size_t)TL;
+  if (carry) w[len++] = 1;
+  const int sign = a->sign;
+  set_mantissa(t, w);
+  t->sign = sign;
+  t->exp = exp;
+  t->len = len;
+}
+
+/* bigfloat/addsub.c:119-171: |a| - |b| (|a| >= |b|) in the window, sign of a,
+ * zero top words stripped; an all-zero result becomes the canonical zero. */
+static void ref_usub(ref_bigfloat *t, const ref_bigfloat *a, const ref_bigfloat *b, size_t p) {
+  int64_t bot, exp;
+  const int64_t TL = add_window(a, b, p, &bot, &exp);
+  uint32_t *w = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(TL + 1));
+  int64_t borrow = 0;
+  for (int64_t c = 0; c < TL; c++) {
+    int64_t v = (int64_t)word_at(a, bot + c) - (int64_t)word_at(b, bot + c) - borrow;
+    borrow = v < 0;
+    w[c] = (uint32_t)(v + (borrow ? ((int64_t)1 << 32) : 0));
+  }
+  size_t len = (size_t)TL;
+  while (len > 0 && w[len - 1] == 0) len--;
+  const int sign = a->sign;
+  set_mantissa(t, w);
+  t->sign = sign;
+  t->exp = exp;
+  t->len = len;
+  if (len == 0) ref_zero(t);
+}
+
+static void ref_neg(ref_bigfloat *t) { /* bigfloat/neg.c */
+  if (t->len) t->sign = !t->sign;
+}
+
+/* bigfloat/addsub.c:173-198 */
+void ref_bigfloat_add(ref_bigfloat *t, const ref_bigfloat *a, const ref_bigfloat *b, size_t p) {
+  if (a->sign == b->sign) {
+    ref_uadd(t, a, b, p);
+  } else if (ref_bigfloat_ucmp(a, b) > 0) {
+    ref_usub(t, a, b, p);
+  } else {
+    ref_usub(t, b, a, p);
+    ref_neg(t);
+  }
+}
+
+/* bigfloat/addsub.c:200-225 */
+void ref_bigfloat_sub(ref_bigfloat *t, const ref_bigfloat *a, const ref_bigfloat *b, size_t p) {
+  if (a->sign != b->sign) {
+    ref_uadd(t, a, b, p);
+  } else if (ref_bigfloat_ucmp(a, b) > 0) {
+    ref_usub(t, a, b, p);
+  } else {
+    ref_usub(t, b, a, p);
+    ref_neg(t);
+  }
+}
+
+/* bigfloat/rcp.c:30-70: the two-word starting value of the Newton iteration
+ * from the top three words of a, in double arithmetic exactly as the
+ * reference writes it (UINT32_MAX, not 2^32, as the word weight). */
+static void rcp_start(ref_bigfloat *t, const ref_bigfloat *a) {
+  int64_t e = a->exp;
+  size_t AL = a->len;
+  const uint32_t *AT = a->coef;
+  if (AL > 3) {
+    e += (int64_t)(AL - 3);
+    AT += AL - 3;
+    AL = 3;
+  }
+  const double U = (double)UINT32_MAX;
+  double val = AT[0];
+  if (AL >= 2) val += (double)AT[1] * U;
+  if (AL >= 3) val += (double)AT[2] * U * U;
+  val = 1. / val;
+  e = -e;
+  while (val < U) {
+    val *= U;
+    e--;
+  }
+  const uint64_t v = (uint64_t)val;
+  uint32_t *w = (uint32_t *)malloc(2 * sizeof(uint32_t));
+  w[0] = (uint32_t)(v % UINT32_MAX);
+  w[1] = (uint32_t)(v / UINT32_MAX);
+  const int sign = a->sign;
+  set_mantissa(t, w);
+  t->sign = sign;
+  t->len = 2;
+  t->exp = e;
+}
+
+/* bigfloat/rcp.c:15-101: Newton's iteration r1 = r0 - (r0 x - 1) r0 at
+ * precision p, on the reciprocal at precision p/2 + 1 (1 -> 0, 2 -> 1). */
+void ref_bigfloat_rcp(ref_bigfloat *t, const ref_bigfloat *a, size_t p) {
+  if (a->len == 0) {
+    fprintf(stderr, "Divide by Zero\n");
+    abort();
+  }
+  if (p == 0) {
+    rcp_start(t, a);
+    return;
+  }
+  size_t s = p / 2 + 1;
+  if (p == 1) s = 0;
+  if (p == 2) s = 1;
+  ref_bigfloat T = {NULL, 0, 0, 1}, one = {NULL, 0, 0, 1}, t1 = {NULL, 0, 0, 1}, t2 = {NULL, 0, 0, 1},
+               t3 = {NULL, 0, 0, 1};
+  ref_bigfloat_rcp(&T, a, s);
+  uint32_t w1 = 1;
+  one.coef = &w1;
+  one.len = 1;
+  ref_bigfloat_mul(&t1, a, &T, p);
+  ref_bigfloat_sub(&t2, &t1, &one, p);
+  ref_bigfloat_mul(&t3, &t2, &T, p);
+  ref_bigfloat_sub(t, &T, &t3, p);
+  free(T.coef);
+  free(t1.coef);
+  free(t2.coef);
+  free(t3.coef);
+}
+
+/* bigfloat/div.c:14-27: a * rcp(b) at precision p (0: a->len + b->len). */
+void ref_bigfloat_div(ref_bigfloat *t, const ref_bigfloat *a, const ref_bigfloat *b, size_t p) {
+  if (p == 0) p = a->len + b->len;
+  ref_bigfloat r = {NULL, 0, 0, 1};
+  ref_bigfloat_rcp(&r, b, p);
+  ref_bigfloat_mul(t, a, &r, p);
+  free(r.coef);
+}

@@ -61,6 +61,14 @@ int ref_fft_mul_bits(uint32_t *CT, size_t CL, const uint32_t *AT, size_t AL, con
                      int bits, double *margin);
 int ref_fft_mul(uint32_t *CT, size_t CL, const uint32_t *AT, size_t AL, const uint32_t *BT, size_t BL);
 void ref_bigfloat_mul(ref_bigfloat *target, const ref_bigfloat *a, const ref_bigfloat *b, size_t p);
+/* bigfloat/addsub.c, rcp.c, div.c: sums, differences, Newton reciprocal and
+ * quotient with the reference's truncation rules. target's old mantissa is
+ * freed (the reference leaks it or reallocs it in place). */
+int ref_bigfloat_ucmp(const ref_bigfloat *a, const ref_bigfloat *b);
+void ref_bigfloat_add(ref_bigfloat *target, const ref_bigfloat *a, const ref_bigfloat *b, size_t p);
+void ref_bigfloat_sub(ref_bigfloat *target, const ref_bigfloat *a, const ref_bigfloat *b, size_t p);
+void ref_bigfloat_rcp(ref_bigfloat *target, const ref_bigfloat *a, size_t p);
+void ref_bigfloat_div(ref_bigfloat *target, const ref_bigfloat *a, const ref_bigfloat *b, size_t p);
 
 /* Worker threads used by the parallel regions (OpenMP). */
 int ref_num_threads(void);

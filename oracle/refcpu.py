@@ -58,8 +58,13 @@ def lib():
         L.ref_fft_mul_bits.argtypes = [_u32p, sz, _u32p, sz, _u32p, sz, ctypes.c_int, _f64p]
         L.ref_fft_mul_bits.restype = ctypes.c_int
         L.ref_fft_mul.argtypes = [_u32p, sz, _u32p, sz, _u32p, sz]; L.ref_fft_mul.restype = ctypes.c_int
-        L.ref_bigfloat_mul.argtypes = [ctypes.POINTER(RefBigfloat)] * 3 + [sz]
-        L.ref_bigfloat_mul.restype = None
+        for name in ("ref_bigfloat_mul", "ref_bigfloat_add", "ref_bigfloat_sub", "ref_bigfloat_div"):
+            getattr(L, name).argtypes = [ctypes.POINTER(RefBigfloat)] * 3 + [sz]
+            getattr(L, name).restype = None
+        L.ref_bigfloat_rcp.argtypes = [ctypes.POINTER(RefBigfloat)] * 2 + [sz]
+        L.ref_bigfloat_rcp.restype = None
+        L.ref_bigfloat_ucmp.argtypes = [ctypes.POINTER(RefBigfloat)] * 2
+        L.ref_bigfloat_ucmp.restype = ctypes.c_int
         L.ref_num_threads.restype = ctypes.c_int
         L.ref_set_threads.argtypes = [ctypes.c_int]
         L.ref_set_threads.restype = None
@@ -141,9 +146,9 @@ def fft_mul(a, b, bits=0, square=False):
     return c, used, m.value
 
 
-def bigfloat_mul(a, b, p=0):
-    """Reference bigfloat_mul on (words, exp, sign) triples; returns the same."""
-    L = lib()
+def _bigfloat_call(fn, a, b, p):
+    """Run a reference bigfloat operation on (words, exp, sign) triples (b None
+    for the one-operand rcp); returns the target as a triple."""
     keep = []
 
     def mk(t):
@@ -155,19 +160,42 @@ def bigfloat_mul(a, b, p=0):
         bf.exp, bf.len, bf.sign = e, w.size, s
         return bf
 
-    A, B = mk(a), mk(b)
-    if a is b:
-        B = A
+    A = mk(a)
+    B = None if b is None else (A if b is a else mk(b))
     T = RefBigfloat()
     T.coef = ctypes.cast(None, _u32p)
     T.len, T.exp, T.sign = 0, 0, 1
-    L.ref_bigfloat_mul(ctypes.byref(T), ctypes.byref(A), ctypes.byref(B), p)
+    if B is None:
+        fn(ctypes.byref(T), ctypes.byref(A), p)
+    else:
+        fn(ctypes.byref(T), ctypes.byref(A), ctypes.byref(B), p)
     words = np.ctypeslib.as_array(T.coef, shape=(T.len,)).copy() if T.len else np.zeros(0, np.uint32)
     libc = ctypes.CDLL(None)
     libc.free.argtypes = [ctypes.c_void_p]
     if T.coef:
         libc.free(ctypes.cast(T.coef, ctypes.c_void_p))
     return words, T.exp, T.sign
+
+
+def bigfloat_mul(a, b, p=0):
+    """Reference bigfloat_mul on (words, exp, sign) triples; returns the same."""
+    return _bigfloat_call(lib().ref_bigfloat_mul, a, b, p)
+
+
+def bigfloat_add(a, b, p=0):
+    return _bigfloat_call(lib().ref_bigfloat_add, a, b, p)
+
+
+def bigfloat_sub(a, b, p=0):
+    return _bigfloat_call(lib().ref_bigfloat_sub, a, b, p)
+
+
+def bigfloat_rcp(a, p):
+    return _bigfloat_call(lib().ref_bigfloat_rcp, a, None, p)
+
+
+def bigfloat_div(a, b, p=0):
+    return _bigfloat_call(lib().ref_bigfloat_div, a, b, p)
 
 
 def num_threads():

@@ -111,7 +111,7 @@ def test_batch_graph_replays(bc):
     batch = bc.Batch(dC, dA, dB, c_off, cl, a_off, al, b_off, bl, **extra)
     try:
         for rep in range(3):
-            bc.check(bc.lib().bcg_memcpy_h2d(dC.ptr, np.zeros(dC.n, np.uint32).ctypes.data, 4 * dC.n), "clear")
+            dC.zero()
             worst = batch.run()
             C = dC.to_host()
             for i, w in enumerate(want):

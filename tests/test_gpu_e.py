@@ -45,7 +45,7 @@ def test_fraction_division(bc, T, nw):
     n = max(1, (pm.bit_length() + 31) // 32)
     F = bc.div_fraction(int_to_words(pm, n), Q, nw)
     want = (pm << (32 * nw)) // q
-    assert abs(words_to_int(F) - want) <= 8
+    assert words_to_int(F) == want
 
 
 def e_digits_exact(D):
@@ -118,3 +118,39 @@ def test_e_digits_1e9_md5(bc, golden):
     import hashlib
     txt, ms = bc.e_digits(10 ** 9, raw=True)
     assert hashlib.md5(txt).hexdigest() == golden["e_md5"]["1000000000"]
+
+
+def test_error_paths_leave_the_pool_consistent(bc, golden):
+    # every device allocation of the e pipeline failed in turn (binary
+    # splitting levels, division, radix conversion): the call reports the
+    # error, and the next call through the same memory pool is still exact
+    import ctypes
+    import hashlib
+    L = bc.lib()
+    buf = np.empty(10 ** 4 + 3, dtype=np.uint8)
+    ms = (ctypes.c_double * 4)()
+    failed = 0
+    for nth in range(1, 400):
+        bc.check(L.bcg_test_fail_alloc(nth), "fail_alloc")
+        rc = L.bcg_e_digits_ex(10 ** 4, buf.ctypes.data_as(ctypes.c_char_p), buf.size, ms, 0)
+        bc.check(L.bcg_test_fail_alloc(0), "fail_alloc")
+        if rc == 0:
+            break
+        failed += 1
+        assert rc < 0
+        txt, _ = bc.e_digits(10 ** 4)
+        assert hashlib.md5(txt.encode()).hexdigest() == golden["e_md5"]["10000"], nth
+    assert failed >= 10 and rc == 0
+    # radix conversion alone, on a random fraction
+    rng = np.random.default_rng(5)
+    F = rng.integers(0, 2**32, 4000, dtype=np.uint64).astype(np.uint32)
+    want = bc.radix(F, 4000)
+    dF, dD = bc.DeviceWords.from_host(F), bc.DeviceWords(4000)
+    for nth in range(1, 400):
+        bc.check(L.bcg_test_fail_alloc(nth), "fail_alloc")
+        rc = L.bcg_radix_dev(dF.ptr, 4000, 4000, dD.ptr)
+        bc.check(L.bcg_test_fail_alloc(0), "fail_alloc")
+        if rc == 0:
+            break
+        assert np.array_equal(bc.radix(F, 4000), want), nth
+    assert rc == 0 and np.array_equal(dD.to_host(), want)

@@ -141,3 +141,33 @@ def test_tft_round_trip_and_definition(bc, k, ln):
     assert np.abs(z[:ln] - full[:ln]).max() < 1e-9 * n and not z[ln:].any()
     bc.tft_inverse(z, ln, k)
     assert np.abs(z[:ln] - n * x[:ln]).max() < 1e-7 * n and not z[ln:].any()
+
+
+def test_twiddle_tables_through_the_library(bc):
+    # fft/table.c:211-234 through the library's exported globals: the slot
+    # count of the latest fft_ensure_table call, table_select's -1 past it, and
+    # table values e^{+2 pi i c/L} (c < L/2) in slot _table_select(L/2)
+    import ctypes
+    import math
+    L = bc.lib()
+    size = ctypes.c_int.in_dll(L, "twiddle_table_size")
+    tabs = (ctypes.c_void_p * 125).in_dll(L, "twiddle_table")
+    L.fft_free_table.restype = None
+    L.fft_ensure_table(3 << 12)
+    # k stops at 8192 (2k = 16384 > 12288): slots 0 .. _table_select(8192) = 24
+    assert size.value == 25
+    assert L.table_select(4096) == 22 and L.table_select(6144) == 23 and L.table_select(8192) == 24
+    for Lc, slot in [(4, 0), (6, 1), (64, 8), (96, 9), (8192, 22), (12288, 23)]:
+        assert L.table_select(Lc // 2) == slot
+        t = np.ctypeslib.as_array(ctypes.cast(tabs[slot], ctypes.POINTER(ctypes.c_double)), shape=(Lc,))
+        z = t[0::2] + 1j * t[1::2]
+        c = np.arange(Lc // 2)
+        want = np.array([complex(math.cos(2 * math.pi * k / Lc), math.sin(2 * math.pi * k / Lc)) for k in c])
+        assert np.max(np.abs(z - want)) < 4e-16 * max(1.0, math.log2(Lc)), Lc
+        assert z[0] == 1
+    # a smaller call resets the size (fft/table.c:227), so bigger slots read as absent
+    L.fft_ensure_table(100)
+    assert size.value == 11
+    assert L.table_select(64) == 10 and L.table_select(4096) == ctypes.c_size_t(-1).value
+    L.fft_free_table()
+    assert size.value == 0 and all(tabs[i] is None for i in range(125))

@@ -22,9 +22,12 @@ import bluecrunch_amd as bc
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = "/root/reference"
 CFLAGS = ["-std=c11", "-Wall", "-Werror", "-O3", "-no-pie", "-mavx", "-funroll-loops"]
-# the files INTEGRATION.md drops from the reference build (replaced by the library)
+# the files INTEGRATION.md drops from the reference build (replaced by the
+# library): the multiply path, and optionally also sum, reciprocal and quotient
 DROPPED = {"fft/fft.c", "fft/baileys.c", "fft/tft.c", "fft/pointwise.c", "fft/convert.c", "fft/table.c",
            "bigfloat/mul.c"}
+DROPPED_ARITH = DROPPED | {"bigfloat/addsub.c", "bigfloat/rcp.c", "bigfloat/div.c"}
+ARITH = {"bigfloat_add", "bigfloat_sub", "bigfloat_rcp", "bigfloat_div"}
 # symbols of the multiply path every program must take from the library
 HOT = {"bigfloat_mul", "_fft_mul", "_karatzuba_mul", "_basecase_mul", "fft_forward", "fft_inverse",
        "fft_pointwise", "int_to_fft", "fft_to_int", "fft_ensure_table", "fft_free_table", "fft_length",
@@ -35,12 +38,12 @@ pytestmark = pytest.mark.skipif(not os.path.exists(os.path.join(REF, "main.c")) 
                                 reason="reference tree or gcc not present")
 
 
-def _kept_sources():
+def _kept_sources(dropped=DROPPED):
     out = []
     for d in ("bigfloat", "fft", "util"):
         for f in sorted(os.listdir(os.path.join(REF, d))):
             rel = f"{d}/{f}"
-            if f.endswith(".c") and rel not in DROPPED:
+            if f.endswith(".c") and rel not in dropped:
                 out.append(rel)
     return out
 
@@ -66,7 +69,7 @@ def objects(tmp_path_factory):
     errors = {rel: r.stderr for rel, _, r in res if r.returncode}
     assert not errors, errors
     objs = {rel: obj for rel, obj, _ in res}
-    return d, [objs[s] for s in _kept_sources()], {p: objs[p] for p in progs}
+    return d, objs, {p: objs[p] for p in progs}
 
 
 def _link(d, prog_obj, lib_objs, name):
@@ -87,10 +90,12 @@ def _defined(obj):
     return {line.split()[-1] for line in out.splitlines() if line.strip()}
 
 
-def test_kept_objects_define_no_hot_path_symbol(objects):
-    _, lib_objs, _ = objects
-    for o in lib_objs:
-        assert not (_defined(o) & HOT), (o, _defined(o) & HOT)
+@pytest.mark.parametrize("variant", ["multiply", "arith"])
+def test_kept_objects_define_no_replaced_symbol(objects, variant):
+    _, objs, _ = objects
+    dropped, hot = (DROPPED, HOT) if variant == "multiply" else (DROPPED_ARITH, HOT | ARITH)
+    for rel in _kept_sources(dropped):
+        assert not (_defined(objs[rel]) & hot), (rel, _defined(objs[rel]) & hot)
 
 
 @pytest.mark.parametrize("prog", ["main.c", "bench.c", "test/test_regression.c", "test/test_bigfloat_exp.c",
@@ -98,14 +103,19 @@ def test_kept_objects_define_no_hot_path_symbol(objects):
                                   "test/test_conversion.c", "test/test_tft.c", "test/test_baileys.c",
                                   "test/test_fft_benchmark.c", "test/test_radix_conv.c",
                                   "test/test_bigfloat_log.c", "test/test_test.c"])
-def test_program_links_against_the_library(objects, prog):
-    d, lib_objs, progs = objects
-    exe, r = _link(d, progs[prog], lib_objs, prog.replace("/", "_")[:-2])
+@pytest.mark.parametrize("variant", ["multiply", "arith"])
+def test_program_links_against_the_library(objects, prog, variant):
+    d, objs, progs = objects
+    dropped = DROPPED if variant == "multiply" else DROPPED_ARITH
+    lib_objs = [objs[r] for r in _kept_sources(dropped)]
+    exe, r = _link(d, progs[prog], lib_objs, prog.replace("/", "_")[:-2] + "_" + variant)
     assert r.returncode == 0, r.stderr
     und = _undefined(exe)
     provided = set(bc.exported_symbols())
     # what the program takes from the library is what the library exports
-    assert (und & HOT) <= provided
+    assert (und & (HOT | ARITH)) <= provided
+    if variant == "arith" and prog == "main.c":  # e's division and sums from the library too
+        assert {"bigfloat_div", "bigfloat_add"} <= und
     if prog == "main.c":  # e_BSR, rcp, div, exp and radix all multiply through the library
         assert {"bigfloat_mul", "fft_ensure_table", "fft_free_table", "bigfloat_mului", "bigfloat_mulu"} <= und
     if prog == "bench.c":

@@ -85,11 +85,13 @@ def plan_split(bc, n):
     return {"L": list(L), "C": list(C)}
 
 
-def e_pipeline(bc, digits, compare_digits):
+def e_pipeline(bc, digits, compare_digits, cpu_digits):
     """e = 2.71828... to `digits` decimals on the GPU (batched binary splitting,
     Newton division, radix conversion), wall time around the library call, MD5
     of the text against the reference's md5sums (tests/golden/vectors.json).
-    At compare_digits the binary splitting also runs one call per product."""
+    At compare_digits the binary splitting also runs one call per product. At
+    cpu_digits the oracle's restatement of the reference's e pipeline runs on
+    the host cores beside a GPU run of the same size."""
     import hashlib
 
     with open(os.path.join(ROOT, "tests", "golden", "vectors.json")) as f:
@@ -115,7 +117,10 @@ def e_pipeline(bc, digits, compare_digits):
            # digit size and margin at the longest transform
            "products": st["products"], "worst_margin": st["worst_margin"], "max_points": st["max_points"],
            "bits_at_max": st["bits_at_max"], "margin_at_max": st["margin_at_max"]}
+    if cpu_digits:
+        out["cpu_baseline"] = e_cpu_baseline(bc, cpu_digits, gold)
     if compare_digits:
+        print(f"bench: per-multiply binary splitting at {compare_digits} digits", file=sys.stderr, flush=True)
         _, mb = bc.e_digits(compare_digits)
         _, mp = bc.e_digits(compare_digits, per_multiply=True)
         out["bsr_compare"] = {"digits": compare_digits, "batched_bsr_ms": mb["bsr_ms"],
@@ -123,6 +128,39 @@ def e_pipeline(bc, digits, compare_digits):
                               "speedup": mp["bsr_ms"] / mb["bsr_ms"] if mb["bsr_ms"] > 0 else None,
                               "batched_total_ms": mb["total_ms"], "per_multiply_total_ms": mp["total_ms"]}
     return out
+
+
+def e_cpu_baseline(bc, digits, gold):
+    """The reference's e pipeline (main.c:111-167) restated in the oracle
+    (oracle/refe.c, OpenMP tasks at the reference's Cilk thresholds) on the
+    host cores, after the twiddle tables main.c:185-209 builds before its timed
+    phases; beside it the GPU pipeline at the same size (first run in this
+    process at that size). Both texts are checked against md5sums."""
+    import hashlib
+    from oracle import refcpu
+
+    L = refcpu.lib()
+    p = 2 * digits // 9 + 10  # main.c:185-197
+    length = 1
+    while length < 3 * p:
+        length *= 2
+    L.ref_ensure_table(min(length, 100663296))
+    print(f"bench: oracle e to {digits} digits on {refcpu.num_threads()} threads", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    txt, ms = refcpu.e_digits(digits)
+    cpu_s = time.perf_counter() - t0
+    cpu_ok = hashlib.md5(txt).hexdigest() == gold.get(str(digits))
+    del txt
+    t0 = time.perf_counter()
+    gtxt, gms = bc.e_digits(digits, raw=True)
+    gpu_s = time.perf_counter() - t0
+    gpu_ok = hashlib.md5(gtxt).hexdigest() == gold.get(str(digits))
+    return {"digits": digits, "cpu_wall_s": cpu_s, "cpu_bsr_ms": ms["bsr_ms"], "cpu_div_ms": ms["div_ms"],
+            "cpu_radix_ms": ms["radix_ms"], "cpu_md5_ok": cpu_ok, "gpu_wall_s": gpu_s, "gpu_md5_ok": gpu_ok,
+            "speedup": cpu_s / gpu_s if gpu_s > 0 else None, "cores": refcpu.num_threads(),
+            "host_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "kind": "port",
+            "sample": f"oracle restatement of main.c e() at {digits} digits (e_BSR, bigfloat_div, bigfloat_add, "
+                      "bigfloat_radix, bigfloat_to_string), one run"}
 
 
 def batch_leg(bc, count, seed):
@@ -275,8 +313,10 @@ def main():
     ap.add_argument("--no-square", action="store_true", help="skip the squaring runs (profiler runs)")
     ap.add_argument("--e-digits", type=int, default=10 ** 8, help="digits of e for the e leg (0: skip)")
     ap.add_argument("--batch", type=int, default=4096, help="products in the batched leg (0: skip)")
-    ap.add_argument("--e-compare-digits", type=int, default=10 ** 6,
+    ap.add_argument("--e-compare-digits", type=int, default=10 ** 8,
                     help="digits of e for the batched vs per-multiply binary-splitting comparison (0: skip)")
+    ap.add_argument("--e-cpu-digits", type=int, default=10 ** 7,
+                    help="digits of e for the CPU baseline of the e pipeline (oracle on the host cores; 0: skip)")
     ap.add_argument("--split", default=None,
                     help="force the pass split, e.g. 64,256,4096 (skips the autotuner); for profiler runs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -414,7 +454,10 @@ def main():
             bc.mul(a, b)
         host_ms = (time.perf_counter() - t4) / 3 * 1e3
 
-    e_leg = e_pipeline(bc, args.e_digits, args.e_compare_digits) if args.e_digits else None
+    print("bench: e leg", file=sys.stderr, flush=True)
+    e_leg = e_pipeline(bc, args.e_digits, args.e_compare_digits, 0 if args.no_cpu else args.e_cpu_digits) \
+        if args.e_digits else None
+    print("bench: batch leg", file=sys.stderr, flush=True)
     b_leg = batch_leg(bc, args.batch, 7) if args.batch else None
 
     cpu = None

@@ -666,6 +666,9 @@ def e_digits(digits, per_multiply=False, raw=False):
     check(lib().bcg_e_digits_ex(digits, buf.ctypes.data_as(ctypes.c_char_p), digits + 3, ms,
                                 1 if per_multiply else 0), "bcg_e_digits")
     times = {"bsr_ms": ms[0], "div_ms": ms[1], "radix_ms": ms[2], "total_ms": ms[3]}
+    # the text ends at its terminator: digits + 2 characters, fewer for a few
+    # small digit counts (as the reference's e.txt)
+    n = digits + 2 if digits >= 100 else buf[:digits + 3].tobytes().index(b"\0")
     if raw:
-        return memoryview(buf)[:digits + 2].toreadonly(), times
-    return buf[:digits + 2].tobytes().decode(), times
+        return memoryview(buf)[:n].toreadonly(), times
+    return buf[:n].tobytes().decode(), times

@@ -172,7 +172,9 @@ int bcg_div_fraction_dev(const uint32_t *P, size_t pl, const uint32_t *Q, size_t
  * (the role of bigfloat_radix, bigfloat/radix.c:321-360). */
 int bcg_radix_dev(const uint32_t *F, size_t nf, size_t groups, uint32_t *digits);
 /* The whole e pipeline on the GPU: out = "2." + `digits` decimal digits of e,
- * truncated, NUL-terminated (the e.txt of main.c:111-167); cap >= digits + 3.
+ * truncated, NUL-terminated: the e.txt of main.c:111-167, which for a few
+ * small digit counts (1-4, 10-12, 19, 20, 28) holds fewer digits, as the
+ * reference's bigfloat_to_string prints them; cap >= digits + 3.
  * ms (optional, 4 entries): binary splitting, division, radix conversion,
  * total wall milliseconds. */
 int bcg_e_digits(size_t digits, char *out, size_t cap, double *ms);

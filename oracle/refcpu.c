@@ -185,8 +185,19 @@ static void inv_rec(cplx *T, size_t L) {
   }
 }
 
+/* Inside a parallel region (a task of the e pipeline's trees) the transform's
+ * own tasks go to that team, as Cilk's spawns go to its workers; outside one
+ * a team is started. */
+static int in_team(void) {
+#ifdef _OPENMP
+  return omp_in_parallel();
+#else
+  return 1;
+#endif
+}
+
 static void run_top(void (*fn)(cplx *, size_t), cplx *T, size_t L) {
-  if (L < SPAWN_LENGTH) {
+  if (L < SPAWN_LENGTH || in_team()) {
     fn(T, L);
     return;
   }
@@ -213,6 +224,16 @@ void ref_fft_inverse(double *z, size_t length) {
 void ref_fft_pointwise(double *a, const double *b, size_t length) {
   cplx *A = (cplx *)a;
   const cplx *B = (const cplx *)b;
+  if (length >= SPAWN_LENGTH && in_team()) {
+#pragma omp taskloop num_tasks(64)
+    for (size_t i = 0; i < length; i++) {
+      double re = A[i].re * B[i].re - A[i].im * B[i].im;
+      double im = A[i].im * B[i].re + A[i].re * B[i].im;
+      A[i].re = re;
+      A[i].im = im;
+    }
+    return;
+  }
 #pragma omp parallel for schedule(static) if (length >= SPAWN_LENGTH)
   for (size_t i = 0; i < length; i++) {
     double re = A[i].re * B[i].re - A[i].im * B[i].im;
@@ -437,7 +458,12 @@ int ref_fft_mul_bits(uint32_t *CT, size_t CL, const uint32_t *AT, size_t AL, con
   ref_int_to_fft(Ta, length, AT, AL, bits);
   if (!square) ref_int_to_fft(Tb, length, BT, BL, bits);
   ref_ensure_table(length);
-  if (!square && length >= SPAWN_LENGTH) {
+  if (!square && length >= SPAWN_LENGTH && in_team()) {
+#pragma omp task
+    fwd_rec((cplx *)Ta, length);
+    fwd_rec((cplx *)Tb, length);
+#pragma omp taskwait
+  } else if (!square && length >= SPAWN_LENGTH) {
 #pragma omp parallel
 #pragma omp single
     {

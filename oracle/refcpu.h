@@ -70,6 +70,19 @@ void ref_bigfloat_sub(ref_bigfloat *target, const ref_bigfloat *a, const ref_big
 void ref_bigfloat_rcp(ref_bigfloat *target, const ref_bigfloat *a, size_t p);
 void ref_bigfloat_div(ref_bigfloat *target, const ref_bigfloat *a, const ref_bigfloat *b, size_t p);
 
+/* refe.c: the e pipeline of main.c:38-167 and what it needs from
+ * bigfloat/{mul,exp,radix,string}.c. */
+void ref_bigfloat_mului(ref_bigfloat *a, uint32_t b);
+void ref_bigfloat_mulu(ref_bigfloat *t, const ref_bigfloat *a, uint32_t b);
+void ref_bigfloat_exp(ref_bigfloat *t, const ref_bigfloat *a, uint64_t n);
+void ref_bigfloat_radix(ref_bigfloat *t, const ref_bigfloat *a);
+size_t ref_bigfloat_to_string10(char *s, const ref_bigfloat *v, size_t digits);
+size_t ref_e_terms(size_t p);
+void ref_e_bsr(ref_bigfloat *P, ref_bigfloat *Q, uint32_t a, uint32_t b);
+/* e.txt ("2." + digits decimals) into out; returns its length. ms: 4 entries
+ * (binary splitting, division, radix + text, total) or NULL. */
+size_t ref_e_digits(size_t digits, char *out, size_t cap, double *ms);
+
 /* Worker threads used by the parallel regions (OpenMP). */
 int ref_num_threads(void);
 void ref_set_threads(int n); /* OpenMP threads of later calls (n > 0) */

@@ -65,6 +65,10 @@ def lib():
         L.ref_bigfloat_rcp.restype = None
         L.ref_bigfloat_ucmp.argtypes = [ctypes.POINTER(RefBigfloat)] * 2
         L.ref_bigfloat_ucmp.restype = ctypes.c_int
+        L.ref_e_terms.argtypes = [sz]
+        L.ref_e_terms.restype = sz
+        L.ref_e_digits.argtypes = [sz, ctypes.c_char_p, sz, _f64p]
+        L.ref_e_digits.restype = sz
         L.ref_num_threads.restype = ctypes.c_int
         L.ref_set_threads.argtypes = [ctypes.c_int]
         L.ref_set_threads.restype = None
@@ -196,6 +200,19 @@ def bigfloat_rcp(a, p):
 
 def bigfloat_div(a, b, p=0):
     return _bigfloat_call(lib().ref_bigfloat_div, a, b, p)
+
+
+def e_digits(digits):
+    """The reference's e.txt for `digits` decimals ("2." + digits), computed by
+    the restated e pipeline (oracle/refe.c); returns (bytes, timings dict)."""
+    buf = ctypes.create_string_buffer(digits + 32)
+    ms = (ctypes.c_double * 4)()
+    n = lib().ref_e_digits(digits, buf, digits + 32, ms)
+    return buf.raw[:n], {"bsr_ms": ms[0], "div_ms": ms[1], "radix_ms": ms[2], "total_ms": ms[3]}
+
+
+def e_terms(p):
+    return lib().ref_e_terms(p)
 
 
 def num_threads():

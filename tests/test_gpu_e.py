@@ -154,3 +154,10 @@ def test_error_paths_leave_the_pool_consistent(bc, golden):
             break
         assert np.array_equal(bc.radix(F, 4000), want), nth
     assert rc == 0 and np.array_equal(dD.to_host(), want)
+
+
+def test_e_text_equals_the_oracle_for_small_digit_counts(bc, oracle):
+    # e.txt as the reference writes it, including its short texts
+    for D in list(range(1, 60)) + [99, 100, 101, 997, 2000]:
+        txt, _ = bc.e_digits(D)
+        assert txt.encode() == oracle.e_digits(D)[0], D

@@ -26,7 +26,7 @@ _vp = ctypes.c_void_p
 EXPORTS = {
     "bluecrunch_gpu.h": [
         "bcg_init", "bcg_shutdown", "bcg_last_error", "bcg_stream", "bcg_sync", "bcg_malloc", "bcg_free",
-        "bcg_memcpy_h2d", "bcg_memcpy_d2h", "bcg_memcpy_d2d", "bcg_mul_dev", "bcg_mul_host", "bcg_mul_dev_bits", "bcg_basecase_dev",
+        "bcg_memcpy_h2d", "bcg_memcpy_d2h", "bcg_memcpy_d2d", "bcg_mul_dev", "bcg_mul_host", "bcg_mul_dev_bits", "bcg_mul_dev_noalloc", "bcg_basecase_dev",
         "bcg_mul_batch_dev", "bcg_muladd_dev", "bcg_muladd_batch_dev", "bcg_batch_create", "bcg_batch_run", "bcg_batch_destroy", "bcg_fft_forward_dev", "bcg_fft_inverse_dev", "bcg_fft_pointwise_dev",
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
@@ -98,6 +98,7 @@ def lib():
     mul_args = [_vp, _sz, _vp, _sz, _vp, _sz]
     L.bcg_mul_dev.argtypes = mul_args + [_vp, ctypes.POINTER(MulInfo)]
     L.bcg_mul_dev_bits.argtypes = mul_args + [i, _vp, ctypes.POINTER(MulInfo)]
+    L.bcg_mul_dev_noalloc.argtypes = mul_args + [_vp, ctypes.POINTER(MulInfo)]
     L.bcg_mul_host.argtypes = mul_args + [ctypes.POINTER(MulInfo)]
     L.bcg_basecase_dev.argtypes = mul_args + [_vp]
     L.bcg_basecase_host.argtypes = mul_args

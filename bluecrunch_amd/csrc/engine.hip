@@ -97,8 +97,14 @@ int fail(int code, const char *what, hipError_t e = hipSuccess) {
 
 void ep_trim(size_t keep = 0);  // idle blocks of the e pipeline's pool
 
+// True while bcg_mul_dev_noalloc runs: the engine's buffers are then slices
+// of the static scratch (their sizes are limits), host transfers bypass the
+// pinned ring, and nothing is allocated.
+bool g_fixed = false;
+
 int ensure(DevBuf &b, size_t bytes) {
   if (b.bytes >= bytes) return 0;
+  if (g_fixed) return fail(BCG_E_ARG, "product too large for the allocation-free scratch");
   if (b.p) {
     HIPCHK(hipStreamSynchronize(g.stream));
     HIPCHK(hipFree(b.p));
@@ -191,6 +197,11 @@ int ring_events() {
 // of one chunk overlaps the DMA of the one before.
 int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
   if (!bytes) return 0;
+  if (g_fixed) {  // from the caller's memory, complete on return
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+  }
   constexpr size_t CH = RING_CHUNK;
   char *st;
   if (bytes <= CH) {
@@ -217,6 +228,11 @@ int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
 // one chunk overlapping the DMA of the next.
 int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
   constexpr size_t CH = RING_CHUNK;
+  if (g_fixed) {
+    if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+  }
   if (bytes <= CH) {
     char *st;
     if (int rc = ring_take(bytes ? bytes : 1, &st)) return rc;
@@ -244,6 +260,8 @@ int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
   return 0;
 }
 
+int prepare_noalloc();  // tables and the scratch address of bcg_mul_dev_noalloc
+
 int init_locked(int device) {
   if (g.ready) return 0;
   int n = 0;
@@ -265,7 +283,7 @@ int init_locked(int device) {
   // (a failure here only defers the load to the first launch)
   for (auto f : {preload_mul1, preload_fwd_pow2, preload_fwd_three, preload_inv_pow2, preload_inv_three, preload_mid})
     if (f()) (void)hipGetLastError();
-  return 0;
+  return prepare_noalloc();
 }
 
 #define ENTER()                                   \
@@ -391,6 +409,7 @@ __global__ void k_fill_roots(Roots r, cd *lo, unsigned long long nlo, cd *hi, un
 // values within its 1.8e-16).
 int get_roots(unsigned long long N, Roots *out) {
   auto it = g.roots.find(N);
+  if (it == g.roots.end() && g_fixed) return fail(BCG_E_ARG, "root table not prepared at initialisation");
   if (it == g.roots.end()) {
     RootsDev r;
     r.four_n = 4 * N;
@@ -436,6 +455,7 @@ int get_roots(unsigned long long N, Roots *out) {
 
 int get_ltab(int L, const cd **out) {
   auto it = g.ltab.find(L);
+  if (it == g.ltab.end() && g_fixed) return fail(BCG_E_ARG, "pass table not prepared at initialisation");
   if (it == g.ltab.end()) {
     int n = L / 2 > 0 ? L / 2 : 1;
     std::vector<double> t(2 * n);
@@ -1504,6 +1524,86 @@ __global__ void k_fill_hash(uint32_t *p, unsigned long long n, unsigned long lon
 
 }  // namespace
 
+// ======================================================== allocation-free
+
+namespace {
+
+// Scratch of bcg_mul_dev_noalloc: the engine's per-product buffers as fixed
+// slices of one static device array, for products whose transform has at
+// most NOALLOC_MAX_POINTS points: two transform arrays (16 bytes a point),
+// the margin slots, carry block functions, one product descriptor, counters,
+// and the staging of an aliased output (one-pass and schoolbook products).
+constexpr unsigned long long NOALLOC_MAX_POINTS = 1ull << 20;
+constexpr size_t NA_ARR = (size_t)NOALLOC_MAX_POINTS * sizeof(cd);
+constexpr size_t NA_MARGIN = (size_t)MARGIN_SLOTS * MARGIN_STRIDE * sizeof(unsigned int);
+constexpr size_t NA_BLK = (size_t)64 << 10;
+constexpr size_t NA_DESC = 256;
+constexpr size_t NA_COUNTERS = 256;
+constexpr size_t NA_ZBUF = (size_t)64 << 10;
+constexpr size_t NA_BYTES = 2 * NA_ARR + NA_MARGIN + NA_BLK + NA_DESC + NA_COUNTERS + NA_ZBUF;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+
+// Device address of `scratch` per device (0: not looked up). Host code uses
+// these, never the symbol itself (on the host it is only a stand-in).
+constexpr int NA_MAX_DEVICES = 64;
+unsigned char *g_scratch_ptr[NA_MAX_DEVICES] = {nullptr};
+
+// At initialisation, with the device current: the scratch's address, and
+// every table and pass split a product up to NOALLOC_MAX_POINTS points can
+// use (root tables per length class, per-pass tables per pass length), so that
+// the allocation-free entry point finds them built.
+int prepare_noalloc() {
+  if (g.device < 0 || g.device >= NA_MAX_DEVICES) return fail(BCG_E_ARG, "device index beyond the scratch table");
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return fail(BCG_E_HIP, "scratch address", e);
+  g_scratch_ptr[g.device] = (unsigned char *)ptr;
+  for (unsigned long long b = 32; 2 * b <= NOALLOC_MAX_POINTS; b *= 2) {
+    for (unsigned long long N : {2 * b, 3 * b}) {
+      if (N > NOALLOC_MAX_POINTS) continue;
+      Plan p;
+      if (!factor_cached(N, p)) continue;
+      Roots r;
+      if (int rc = get_roots(N, &r)) return rc;
+      for (const Plan &c : g_cands[N])
+        for (int i = 0; i < c.P; i++) {
+          const cd *t = nullptr;
+          if (int rc = get_ltab(c.L[i], &t)) return rc;
+        }
+    }
+  }
+  return 0;
+}
+
+// The engine's buffers pointed at the scratch for one call (under the engine
+// lock), restored afterwards.
+struct ScratchScope {
+  DevBuf *bufs[7] = {&g.bufA, &g.bufB, &g.margin, &g.blk, &g.desc, &g.counters, &g.zbuf};
+  DevBuf saved[7];
+  explicit ScratchScope(unsigned char *base) {
+    const size_t sizes[7] = {NA_ARR, NA_ARR, NA_MARGIN, NA_BLK, NA_DESC, NA_COUNTERS, NA_ZBUF};
+    size_t off = 0;
+    for (int i = 0; i < 7; i++) {
+      saved[i] = *bufs[i];
+      bufs[i]->p = base + off;
+      bufs[i]->bytes = sizes[i];
+      off += sizes[i];
+    }
+    g_fixed = true;
+  }
+  ~ScratchScope() {
+    for (int i = 0; i < 7; i++) *bufs[i] = saved[i];
+    g_fixed = false;
+  }
+};
+
+}  // namespace
+
 // =================================================================== C ABI
 
 extern "C" {
@@ -1625,6 +1725,23 @@ int bcg_mul_dev_bits(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const
 int bcg_mul_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL, void *stream,
                 bcg_mul_info *info) {
   return bcg_mul_dev_bits(C, CL, A, AL, B, BL, 0, stream, info);
+}
+
+int bcg_mul_dev_noalloc(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                        void *stream, bcg_mul_info *info) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  if (!g.ready) return fail(BCG_E_ARG, "call bcg_init before bcg_mul_dev_noalloc");
+  HIPCHK(hipSetDevice(g.device));
+  unsigned char *base = g_scratch_ptr[g.device];
+  if (!base) return fail(BCG_E_HIP, "allocation-free scratch not set up");
+  if ((AL && !A) || (BL && !B) || (CL && !C)) return fail(BCG_E_ARG, "null pointer");
+  if (AL && BL && AL + BL >= 100) {
+    Plan p;
+    if (int rc = make_plan(AL, BL, 0, p)) return rc;
+    if (p.N > NOALLOC_MAX_POINTS) return fail(BCG_E_ARG, "transform longer than the allocation-free limit (2^20 points)");
+  }
+  ScratchScope scope(base);
+  return mul_dev_locked(C, CL, A, AL, B, BL, 0, pick(stream), info);
 }
 
 int bcg_mul_host(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,

@@ -68,6 +68,20 @@ int bcg_mul_host(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uin
 int bcg_mul_dev_bits(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
                      int bits, void *stream, bcg_mul_info *info);
 
+/* bcg_mul_dev without any allocation: no hipMalloc, hipFree or other
+ * allocating call, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops). No counterpart in the reference. Its buffers
+ * are one static device array of 33,694,208 bytes (32.1 MiB) per GPU, held
+ * until the process ends and set up by bcg_init (which must have run), so the
+ * transform is limited to 2^20 points (operands up to about 400,000 words
+ * each; bcg_plan_query gives the length) and an output that aliases an
+ * operand of a one-pass or schoolbook product to 16,368 words; larger
+ * products fail with BCG_E_ARG. Timing events (bcg_prof_enable) are the one
+ * thing it may still create. Every call on a GPU shares that array: calls that overlap on one
+ * GPU (other threads, other streams) race. Same result as bcg_mul_dev. */
+int bcg_mul_dev_noalloc(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
+                        void *stream, bcg_mul_info *info);
+
 /* Schoolbook product on the GPU (exact), used for CL < 100 like the
  * reference _karatzuba_mul/_basecase_mul branch (bigfloat/mul.c:176-553). */
 int bcg_basecase_dev(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,

@@ -110,8 +110,22 @@ def e_pipeline(bc, digits, compare_digits, cpu_digits):
     wall2 = time.perf_counter() - t0
     warm_ok = hashlib.md5(text2).hexdigest() == md5
     del text2
+    # the same with the default splits for batched classes (table restored after)
+    wall_default = None
+    saved = batch_table(bc)
+    if saved:
+        for n in saved:
+            bc.plan_force_batch(n, -1)
+        t0 = time.perf_counter()
+        text3, _ = bc.e_digits(digits, raw=True)
+        wall_default = time.perf_counter() - t0
+        warm_ok = warm_ok and hashlib.md5(text3).hexdigest() == md5
+        del text3
+        for n, i in saved.items():
+            bc.plan_force_batch(n, i)
     out = {"digits": digits, "wall_s": wall, "bsr_ms": ms["bsr_ms"], "div_ms": ms["div_ms"],
            "radix_ms": ms["radix_ms"], "wall_s_warm": wall2, "warm_equal": warm_ok, "md5": md5,
+           "wall_s_warm_default_splits": wall_default,
            "md5_ok": (md5 == gold[str(digits)]) if str(digits) in gold else None,
            # every FFT product of the run: the worst rounding margin, and the
            # digit size and margin at the longest transform
@@ -161,6 +175,18 @@ def e_cpu_baseline(bc, digits, gold):
             "host_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "kind": "port",
             "sample": f"oracle restatement of main.c e() at {digits} digits (e_BSR, bigfloat_div, bigfloat_add, "
                       "bigfloat_radix, bigfloat_to_string), one run"}
+
+
+def batch_table(bc, lo=64, hi=1 << 26):
+    """{N: candidate index} of the batched length classes with a tuned choice."""
+    out = {}
+    b = 32
+    while 2 * b <= hi:
+        for n in (2 * b, 3 * b):
+            if lo <= n <= hi and len(bc.plan_candidates(n)) > 1:
+                out[n] = bc.plan_current_batch(n)
+        b *= 2
+    return out
 
 
 def batch_leg(bc, count, seed):
@@ -225,6 +251,24 @@ def batch_leg(bc, count, seed):
     tg = (time.perf_counter() - t0) / 3
     same_graph = bool(np.array_equal(dC.to_host(), dC3.to_host()))
     graph.close()
+    # the same replay with the default splits (batched length-class table
+    # cleared for the measurement, then restored)
+    tg_untuned = None
+    saved = batch_table(bc)
+    if saved:
+        for n in saved:
+            bc.plan_force_batch(n, -1)
+        graph = bc.Batch(dC3, dA, dB, c_off, cl, a_off, al, b_off, bl)
+        graph.run()
+        bc.sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            graph.run()
+        bc.sync()
+        tg_untuned = (time.perf_counter() - t0) / 3
+        graph.close()
+        for n, i in saved.items():
+            bc.plan_force_batch(n, i)
     # exact spot checks against Python integers
     from bluecrunch_amd.workload import words_to_int, int_to_words
     C = dC.to_host()
@@ -237,6 +281,7 @@ def batch_leg(bc, count, seed):
         d.free()
     return {"products": count, "seed": seed, "total_operand_words": int(al.sum() + bl.sum()),
             "graph_ms": tg * 1e3, "batched_ms": tb * 1e3, "per_call_ms": tp * 1e3,
+            "graph_ms_default_splits": None if tg_untuned is None else tg_untuned * 1e3,
             "speedup_graph_vs_per_call": tp / tg, "speedup_batched_vs_per_call": tp / tb,
             "products_per_s": count / tg, "worst_margin": max(worst.value, gw),
             "batched_equals_per_call": same, "graph_equals_batched": same_graph, "exact_spot_checks": exact}
@@ -361,6 +406,14 @@ def main():
         t_tune = time.perf_counter()
         bc.autotune(al, bl, reps=2)
         tune_ms = (time.perf_counter() - t_tune) * 1e3
+    # the length classes of batched products (the batch leg, the e pipeline's
+    # tree levels): pass split per class, timed on batches of that class
+    class_table, class_ms = None, None
+    if not args.no_tune and (args.batch or args.e_digits):
+        print("bench: length-class autotuner", file=sys.stderr, flush=True)
+        t_tune = time.perf_counter()
+        class_table = bc.autotune_classes(64, 1 << 26, reps=2)
+        class_ms = (time.perf_counter() - t_tune) * 1e3
     for _ in range(args.warmup):
         info = step()
         if not args.no_square:
@@ -491,6 +544,11 @@ def main():
             "worst_margin": worst,
             "pass_split": plan_split(bc, info["points"]),
             "autotune_ms": tune_ms,
+            "autotune_classes_ms": class_ms,
+            # batched length class -> pass lengths / columns per workgroup the tuner kept
+            "autotune_classes": None if class_table is None else {
+                str(n): "x".join(map(str, L)) + "/" + "x".join(map(str, C[:-1] or [1]))
+                for n, (passes, L, C) in sorted(class_table.items())},
         },
         "square": None if s_info is None else {
             "ms_per_step": sec_sq * 1e3, "digits_per_s": digits / sec_sq, "worst_margin": s_worst,

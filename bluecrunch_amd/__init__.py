@@ -31,7 +31,7 @@ EXPORTS = {
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
-        "bcg_plan_force", "bcg_plan_current", "bcg_plan_digit_bias", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_test_fail_alloc", "bcg_bf_set", "bcg_bf_get", "bcg_bf_free", "bcg_bf_add", "bcg_bf_sub", "bcg_bf_mul", "bcg_bf_rcp", "bcg_bf_div", "bcg_prof_enable", "bcg_prof_select", "bcg_prof_read",
+        "bcg_plan_force", "bcg_plan_current", "bcg_plan_digit_bias", "bcg_autotune_classes", "bcg_plan_force_batch", "bcg_plan_current_batch", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_test_fail_alloc", "bcg_bf_set", "bcg_bf_get", "bcg_bf_free", "bcg_bf_add", "bcg_bf_sub", "bcg_bf_mul", "bcg_bf_rcp", "bcg_bf_div", "bcg_prof_enable", "bcg_prof_select", "bcg_prof_read",
         "bcg_prof_reset", "bcg_stats_reset", "bcg_stats_get",
     ],
     "fft.h": [
@@ -124,6 +124,9 @@ def lib():
     L.bcg_plan_candidates.argtypes = [_sz, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
     L.bcg_plan_force.argtypes = [_sz, i]
     L.bcg_plan_digit_bias.argtypes = [i]
+    L.bcg_autotune_classes.argtypes = [_sz, _sz, i, ctypes.POINTER(i)]
+    L.bcg_plan_force_batch.argtypes = [_sz, i]
+    L.bcg_plan_current_batch.argtypes = [_sz]
     L.bcg_plan_current.argtypes = [_sz]
     L.bcg_autotune.argtypes = [_sz, _sz, i, i, ctypes.POINTER(i), _f64p, i]
     L.bcg_e_terms.argtypes = [_sz]
@@ -480,6 +483,31 @@ def plan_current(n):
     """Index into plan_candidates(n) of the split n-point transforms use now."""
     r = lib().bcg_plan_current(n)
     check(min(r, 0), "plan_current")
+    return r
+
+
+def autotune_classes(lo=64, hi=1 << 26, reps=2):
+    """Tune the pass split of batched products for every length class in
+    [lo, hi]; returns {N: (passes, L, C)} of the classes with a choice."""
+    n = ctypes.c_int()
+    check(lib().bcg_autotune_classes(lo, hi, reps, ctypes.byref(n)), "autotune_classes")
+    out = {}
+    b = 32
+    while 2 * b <= hi:
+        for N in (2 * b, 3 * b):
+            if lo <= N <= hi and len(plan_candidates(N)) > 1:
+                out[N] = plan_candidates(N)[plan_current_batch(N)]
+        b *= 2
+    return out
+
+
+def plan_force_batch(n, index):
+    check(lib().bcg_plan_force_batch(n, index), "plan_force_batch")
+
+
+def plan_current_batch(n):
+    r = lib().bcg_plan_current_batch(n)
+    check(min(r, 0), "plan_current_batch")
     return r
 
 

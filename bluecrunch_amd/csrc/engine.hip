@@ -582,7 +582,9 @@ std::vector<Plan> factor_candidates(unsigned long long N) {
     p.L[0] = (int)N;
     fill_strides(p);
     out.push_back(p);
-    return out;
+    // the largest one-pass class can also run as two passes (64 x 128), which
+    // the length-class tuner weighs against the one-workgroup kernel
+    if (N < 8192) return out;
   }
   for (unsigned long long inner = MAXIN; inner >= (three ? 192ull : 128ull); inner /= 2) {
     if (N % inner) continue;
@@ -611,8 +613,12 @@ std::vector<Plan> factor_candidates(unsigned long long N) {
   return out;
 }
 
-// Tuned choice per transform length: index into factor_candidates(N).
+// Tuned choice per transform length: index into factor_candidates(N), for
+// single products (bcg_autotune, bcg_autotune_classes) and for the products of
+// batched calls (bcg_autotune_classes), which run every pass over a whole
+// length class at once and may prefer another split.
 std::map<unsigned long long, int> g_tuned;
+std::map<unsigned long long, int> g_tuned_batch;
 
 // Untuned choice: a two-pass split whose strided pass still moves >= 8
 // columns (128-byte runs) per row; otherwise the three-pass split with the
@@ -649,7 +655,7 @@ bool factor(unsigned long long N, Plan &p) {
 // Split cache: factor_candidates is only built once per transform length.
 std::map<unsigned long long, std::vector<Plan>> g_cands;
 
-bool factor_cached(unsigned long long N, Plan &p) {
+bool factor_cached(unsigned long long N, Plan &p, bool batched = false) {
   auto it = g_cands.find(N);
   if (it == g_cands.end()) it = g_cands.emplace(N, factor_candidates(N)).first;
   const std::vector<Plan> &c = it->second;
@@ -657,6 +663,10 @@ bool factor_cached(unsigned long long N, Plan &p) {
   int idx = default_candidate(c);
   auto t = g_tuned.find(N);
   if (t != g_tuned.end() && t->second >= 0 && t->second < (int)c.size()) idx = t->second;
+  if (batched) {
+    auto tb = g_tuned_batch.find(N);
+    if (tb != g_tuned_batch.end() && tb->second >= 0 && tb->second < (int)c.size()) idx = tb->second;
+  }
   int bits = p.bits;
   unsigned long long dA = p.dA, dB = p.dB;
   p = c[idx];
@@ -670,7 +680,7 @@ bool factor_cached(unsigned long long N, Plan &p) {
 // 0 in normal use).
 int g_digit_bias = 0;
 
-int make_plan(unsigned long long AL, unsigned long long BL, int force_bits, Plan &best) {
+int make_plan(unsigned long long AL, unsigned long long BL, int force_bits, Plan &best, bool batched = false) {
   best = Plan();
   unsigned long long bestN = 0;
   int bestb = 0;
@@ -693,7 +703,7 @@ int make_plan(unsigned long long AL, unsigned long long BL, int force_bits, Plan
   best.bits = bestb;
   best.dA = digits_of(AL, bestb);
   best.dB = digits_of(BL, bestb);
-  if (!factor_cached(bestN, best)) return fail(BCG_E_ARG, "no pass split for this length");
+  if (!factor_cached(bestN, best, batched)) return fail(BCG_E_ARG, "no pass split for this length");
   return 0;
 }
 
@@ -1867,7 +1877,7 @@ int batch_plan(BatchPlan &bp, uint32_t *C, const uint32_t *A, const uint32_t *B,
         bp.tiny.push_back(desc_of(i, nullptr));
       continue;
     }
-    if (int rc = make_plan(al[i], bl[i], 0, bp.plans[i])) return rc;
+    if (int rc = make_plan(al[i], bl[i], 0, bp.plans[i], true)) return rc;
     if (alias)
       bp.single.push_back(i);
     else if (bp.plans[i].P == 1)
@@ -2406,6 +2416,147 @@ int bcg_autotune(size_t AL, size_t BL, int square, int reps, int *chosen, double
   if (rc) return rc;
   if (chosen) *chosen = best;
   return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Operand words (square products) whose plan has transform length N, the
+// longest such (0 if no square product lands on N).
+size_t words_for_class(unsigned long long N) {
+  size_t lo = 1, hi = (size_t)N;  // 32 w / b digits fill at most 2N slots
+  Plan p;
+  if (make_plan(lo, lo, 0, p) || p.N > N) return 0;
+  while (hi - lo > 1) {
+    const size_t mid = lo + (hi - lo) / 2;
+    if (!make_plan(mid, mid, 0, p) && p.N <= N)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  if (make_plan(lo, lo, 0, p) || p.N != N) return 0;
+  return lo;
+}
+
+// Milliseconds of one batched call's launches over `count` synthetic products
+// of w x w words (plan and upload untimed), averaged over reps.
+int time_batch(size_t w, int count, int reps, uint32_t *dA, uint32_t *dB, uint32_t *dC, double *ms) {
+  std::vector<size_t> off(count), cl(count, 2 * w), ab_off(count), len(count, w);
+  for (int i = 0; i < count; i++) {
+    off[i] = (size_t)i * 2 * w;
+    ab_off[i] = (size_t)i * w;
+  }
+  hipStream_t s = g.stream;
+  BatchPlan bp;
+  if (int rc = batch_plan(bp, dC, dA, dB, nullptr, off.data(), cl.data(), ab_off.data(), len.data(), ab_off.data(),
+                          len.data(), nullptr, nullptr, (size_t)count))
+    return rc;
+  if (int rc = batch_upload(bp, g_bb, s)) return rc;
+  if (int rc = batch_launch(bp, g_bb, s)) return rc;  // warm-up
+  HIPCHK(hipStreamSynchronize(s));
+  hipEvent_t e0 = get_event(), e1 = get_event();
+  int rc = 0;
+  (void)hipEventRecord(e0, s);
+  for (int r = 0; r < reps && !rc; r++) rc = batch_launch(bp, g_bb, s);
+  (void)hipEventRecord(e1, s);
+  (void)hipEventSynchronize(e1);
+  float t = 0;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  g.free_events.push_back(e0);
+  g.free_events.push_back(e1);
+  *ms = t / reps;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcg_autotune_classes(size_t lo, size_t hi, int reps, int *tuned) {
+  ENTER();
+  if (reps < 1) reps = 1;
+  if (hi > ((size_t)1 << 27)) return fail(BCG_E_ARG, "length classes above 2^27 points are not tuned");
+  std::vector<unsigned long long> classes;
+  for (unsigned long long b = 32; 2 * b <= hi; b *= 2)
+    for (unsigned long long N : {2 * b, 3 * b})
+      if (N >= lo && N <= hi) classes.push_back(N);
+  // the largest class sets the scratch: about 2^23 points per timed batch
+  size_t maxw = 0;
+  std::vector<size_t> words(classes.size());
+  std::vector<int> counts(classes.size());
+  for (size_t k = 0; k < classes.size(); k++) {
+    words[k] = words_for_class(classes[k]);
+    const unsigned long long N = classes[k];
+    counts[k] = (int)std::max(1ull, std::min(256ull, (8ull << 20) / N));
+    maxw = std::max(maxw, words[k] * (size_t)counts[k]);
+  }
+  uint32_t *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  HIPCHK(hipMalloc(&dA, maxw * 4 + 4));
+  HIPCHK(hipMalloc(&dB, maxw * 4 + 4));
+  HIPCHK(hipMalloc(&dC, maxw * 8 + 8));
+  LAUNCH(k_fill_hash, grid_for(maxw), 256, g.stream, dA, (unsigned long long)maxw, 11ull);
+  LAUNCH(k_fill_hash, grid_for(maxw), 256, g.stream, dB, (unsigned long long)maxw, 12ull);
+  int rc = 0, done = 0;
+  for (size_t k = 0; k < classes.size() && !rc; k++) {
+    const unsigned long long N = classes[k];
+    if (!words[k]) continue;
+    const int nc = (int)factor_candidates(N).size();
+    if (nc < 2) continue;  // nothing to choose
+    // each product's top word is nonzero in the hash fill only at the end of
+    // the whole array; the plan depends on lengths alone
+    int best = -1;
+    double best_ms = 0;
+    const auto saved = g_tuned_batch.find(N) != g_tuned_batch.end() ? g_tuned_batch[N] : -1;
+    for (int i = 0; i < nc && !rc; i++) {
+      g_tuned_batch[N] = i;
+      double ms = 0;
+      rc = time_batch(words[k], counts[k], reps, dA, dB, dC, &ms);
+      if (!rc && (best < 0 || ms < best_ms)) {
+        best = i;
+        best_ms = ms;
+      }
+    }
+    if (rc || best < 0) {
+      if (saved >= 0) g_tuned_batch[N] = saved;
+      else g_tuned_batch.erase(N);
+    } else {
+      g_tuned_batch[N] = best;
+      done++;
+    }
+  }
+  (void)hipStreamSynchronize(g.stream);
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dC);
+  if (rc) return rc;
+  if (tuned) *tuned = done;
+  return 0;
+}
+
+int bcg_plan_force_batch(size_t N, int index) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  if (index < 0) {
+    g_tuned_batch.erase(N);
+    return 0;
+  }
+  if (index >= (int)factor_candidates(N).size()) return fail(BCG_E_ARG, "candidate index out of range");
+  g_tuned_batch[N] = index;
+  return 0;
+}
+
+int bcg_plan_current_batch(size_t N) {
+  std::lock_guard<std::recursive_mutex> lk(g_lock);
+  Plan p;
+  if (!factor_cached(N, p, true)) return fail(BCG_E_ARG, "no pass split for this length");
+  auto it = g_cands.find(N);
+  for (size_t i = 0; i < it->second.size(); i++) {
+    const Plan &c = it->second[i];
+    if (c.P == p.P && c.L[0] == p.L[0] && c.L[1] == p.L[1] && c.L[2] == p.L[2] && c.C[0] == p.C[0] &&
+        c.C[1] == p.C[1])
+      return (int)i;
+  }
+  return fail(BCG_E_ARG, "no pass split for this length");
 }
 
 size_t bcg_e_terms(size_t words) { return e_terms_of(words); }

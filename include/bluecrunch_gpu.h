@@ -168,6 +168,20 @@ int bcg_plan_current(size_t N);
  * times_ms (optional) receives up to max_out per-candidate milliseconds. */
 int bcg_autotune(size_t AL, size_t BL, int square, int reps, int *chosen, double *times_ms, int max_out);
 
+/* Autotuner over length classes: for every transform length class N
+ * (2^k, 3*2^k) in [lo, hi], times every pass split -- and at 8192 points the
+ * one-workgroup product kernel against the two-pass split -- on a batch of
+ * synthetic products of that class run the way bcg_muladd_batch_dev runs them
+ * (about 2^23 points per batch, at most 256 products), and keeps the fastest
+ * for the batched products of that length (bcg_mul_batch_dev,
+ * bcg_muladd_batch_dev, captured batches, the e pipeline's tree levels).
+ * *tuned (optional) receives the number of classes with a choice.
+ * bcg_plan_force_batch / bcg_plan_current_batch pin and read the choice
+ * (indices into bcg_plan_candidates). */
+int bcg_autotune_classes(size_t lo, size_t hi, int reps, int *tuned);
+int bcg_plan_force_batch(size_t N, int index);
+int bcg_plan_current_batch(size_t N);
+
 /* e = 1 + P/Q (the reference main.c pipeline).
  * bcg_e_terms: number of series terms for `words` words of precision
  * (main.c:38-61). bcg_e_bsr_dev: P = sum_{k=1}^{T} (k+1)...T and Q = T! by

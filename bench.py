@@ -279,7 +279,24 @@ def batch_leg(bc, count, seed):
         exact = exact and bool(np.array_equal(C[int(c_off[i]):int(c_off[i]) + int(cl[i])], want))
     for d in (dA, dB, dC, dC2, dC3):
         d.free()
+    # what one replay must move and compute: every product reads its words and
+    # writes its product; a multi-pass product also moves its transform arrays
+    # (B_mul per product); 5 N log2 N FP64 flops per transform, three per product
+    alg_bytes, flops, onepass_bytes = 0, 0.0, 0
+    for x, y in zip(al, bl):
+        pl = bc.plan(int(x), int(y))
+        n = pl["points"]
+        words = 4 * (2 * int(x) + 2 * int(y))
+        if pl["passes"] == 1:
+            alg_bytes += words
+            onepass_bytes += words
+        else:
+            alg_bytes += 6 * 32 * n + 16 * n + words
+        flops += 3 * 5.0 * n * np.log2(n)
     return {"products": count, "seed": seed, "total_operand_words": int(al.sum() + bl.sum()),
+            "replay_alg_bytes": alg_bytes, "replay_GBps": alg_bytes / tg / 1e9,
+            "replay_hbm_frac": alg_bytes / tg / HBM_PEAK, "replay_fp64_tflops": flops / tg / 1e12,
+            "replay_fp64_frac": flops / tg / 78.6e12, "onepass_word_bytes": onepass_bytes,
             "graph_ms": tg * 1e3, "batched_ms": tb * 1e3, "per_call_ms": tp * 1e3,
             "graph_ms_default_splits": None if tg_untuned is None else tg_untuned * 1e3,
             "speedup_graph_vs_per_call": tp / tg, "speedup_batched_vs_per_call": tp / tb,

@@ -727,6 +727,9 @@ int coef_bits(const Plan &p) {
 // multiples of 32.
 int pass_args(const Plan &p, int i, PassArgs &a) {
   if (p.P > 1 && i == 0 && (p.M[0] % 32 || p.N % 32)) return fail(BCG_E_ARG, "packing pass needs 32 | M and 32 | N");
+  // the packing pass addresses each half of an operand with 32-bit byte offsets
+  if (p.P > 1 && i == 0 && p.N * (unsigned long long)p.bits / 8 >= 0xfffff000ull)
+    return fail(BCG_E_ARG, "transform too long for the packing pass");
   a.N = p.N;
   a.M = p.M[i];
   a.S = (unsigned long long)p.L[i] * p.M[i];

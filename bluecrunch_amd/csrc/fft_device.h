@@ -94,6 +94,22 @@ struct Tile {
 
 __device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
 
+// pad16(row * C + c) for a column c < C, written linear in row when 16 | C, so
+// the per-row terms of an unrolled loop become instruction offsets instead of
+// one address register per row.
+template <int C>
+__device__ __forceinline__ int slot(int row, int c) {
+  if constexpr (C % 16 == 0) return row * (C + C / 16) + c + (c >> 4);
+  else return pad16(row * C + c);
+}
+
+// pad16(q + t * S), linear in t when 16 | S (q >= 0).
+template <int S>
+__device__ __forceinline__ int slot_step(int q, int t) {
+  if constexpr (S % 16 == 0) return pad16(q) + t * (S + S / 16);
+  else return pad16(q + t * S);
+}
+
 __device__ __forceinline__ unsigned brev(unsigned v, int bits) {
   return bits ? (__builtin_bitreverse32(v) >> (32 - bits)) : 0u;
 }
@@ -228,9 +244,39 @@ struct Walk16 {
 
 // ---------------------------------------------------------------- butterflies
 
+// Level twiddle bases of dif_levels / dit_levels: base[l] = tl[q (tstride << l)]
+// (loaded ahead of the data where a kernel wants their latency hidden).
+template <int RB>
+__device__ __forceinline__ void level_bases(const cd *__restrict__ tl, int q, int tstride, cd (&base)[RB]) {
+#pragma unroll
+  for (int l = 0; l < RB; l++) base[l] = tl[q * (tstride << l)];
+}
+
 // RB radix-2 DIF levels on R = 2^RB registers holding rows q + t*(span/R).
 // Level l pairs t, t+half (half = R >> (l+1)); twiddle e^{2 pi i n / (span/2^l)}
 // with n = q + (t mod half)*(span/R) = base_l * w_{2 half}^{t mod half}.
+template <int RB>
+__device__ __forceinline__ void dif_levels(cd (&x)[1 << RB], const cd (&tb)[RB]) {
+  constexpr int R = 1 << RB;
+#pragma unroll
+  for (int l = 0; l < RB; l++) {
+    const int half = R >> (l + 1);
+    const cd base = tb[l];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      if ((t & (2 * half - 1)) < half) {
+        const int j = t & (2 * half - 1);
+        cd a = x[t], b = x[t + half];
+        x[t] = cadd(a, b);
+        cd d = csub(a, b);
+        cd w = (j == 0) ? base : cmul(base, w16(j * (16 / (2 * half))));
+        x[t + half] = cmul(d, w);
+      }
+    }
+  }
+}
+
+// The same with each level's base loaded where the level starts.
 template <int RB>
 __device__ __forceinline__ void dif_levels(cd (&x)[1 << RB], const cd *__restrict__ tl, int q, int tstride) {
   constexpr int R = 1 << RB;
@@ -254,6 +300,27 @@ __device__ __forceinline__ void dif_levels(cd (&x)[1 << RB], const cd *__restric
 
 // Mirror image: radix-2 DIT levels with conjugate twiddles (c = b conj(w);
 // a' = a + c; b' = a - c), bit-reversed input to natural output.
+template <int RB>
+__device__ __forceinline__ void dit_levels(cd (&x)[1 << RB], const cd (&tb)[RB]) {
+  constexpr int R = 1 << RB;
+#pragma unroll
+  for (int l = RB - 1; l >= 0; l--) {
+    const int half = R >> (l + 1);
+    const cd base = tb[l];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      if ((t & (2 * half - 1)) < half) {
+        const int j = t & (2 * half - 1);
+        cd a = x[t];
+        cd w = (j == 0) ? base : cmul(base, w16(j * (16 / (2 * half))));
+        cd c = cmulc(x[t + half], w);
+        x[t] = cadd(a, c);
+        x[t + half] = csub(a, c);
+      }
+    }
+  }
+}
+
 template <int RB>
 __device__ __forceinline__ void dit_levels(cd (&x)[1 << RB], const cd *__restrict__ tl, int q, int tstride) {
   constexpr int R = 1 << RB;
@@ -372,13 +439,13 @@ __device__ __forceinline__ void lds_stage(cd *lds, int span, const cd *__restric
     int row0 = blk * span + q;
     cd x[R];
 #pragma unroll
-    for (int t = 0; t < R; t++) x[t] = lds[pad16((row0 + t * sub) * C + c)];
+    for (int t = 0; t < R; t++) x[t] = lds[slot<C>(row0 + t * sub, c)];
     if (INV)
       dit_levels<RB>(x, tl, q, L / span);
     else
       dif_levels<RB>(x, tl, q, L / span);
 #pragma unroll
-    for (int t = 0; t < R; t++) lds[pad16((row0 + t * sub) * C + c)] = x[t];
+    for (int t = 0; t < R; t++) lds[slot<C>(row0 + t * sub, c)] = x[t];
   }
 }
 
@@ -391,12 +458,12 @@ __device__ __forceinline__ void lds_radix3(cd *lds, int tid) {
     int bf = tid + i * TC::T;
     int c = bf % C;
     int blk = bf / C;
-    int e0 = (blk * 3) * C + c;
-    cd a = lds[pad16(e0)], b = lds[pad16(e0 + C)], d = lds[pad16(e0 + 2 * C)];
+    const int r3 = blk * 3;
+    cd a = lds[slot<C>(r3, c)], b = lds[slot<C>(r3 + 1, c)], d = lds[slot<C>(r3 + 2, c)];
     dft3<INV>(a, b, d);
-    lds[pad16(e0)] = a;
-    lds[pad16(e0 + C)] = b;
-    lds[pad16(e0 + 2 * C)] = d;
+    lds[slot<C>(r3, c)] = a;
+    lds[slot<C>(r3 + 1, c)] = b;
+    lds[slot<C>(r3 + 2, c)] = d;
   }
 }
 

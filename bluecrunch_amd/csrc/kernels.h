@@ -146,32 +146,50 @@ __device__ __forceinline__ double round_store(const PassArgs &a, cd f, unsigned 
 }
 
 // Exact sum of C coefficients v[i] 2^{b i + pos0} (pos0 < 32) as K words
-// (K >= the words the sum spans) into out[0..K) (LDS or global).
+// (K >= the words the sum spans) into out[0..K) (LDS or global). pos0 and b
+// are the same across the wave (callers derive them from the tile, never
+// from the lane), so the shifts and the word emissions are scalar decisions.
 template <int C>
 __device__ __forceinline__ void group_sum_words(const unsigned long long (&v)[C], int pos0, int b, int K,
                                                 uint32_t *out) {
-  // 128-bit accumulator (hi:lo); bit 0 is bit 32 k of the sum
-  unsigned long long lo = 0, hi = 0;
+  // 128-bit accumulator w3:w2:w1:w0; bit 0 is bit 32 k of the sum. Each term
+  // v 2^pos (v < 2^51, pos < 32) spans three words and enters through one
+  // add-with-carry chain.
+  unsigned w0 = 0, w1 = 0, w2 = 0, w3 = 0;
   int pos = pos0, k = 0;
 #pragma unroll
   for (int i = 0; i < C; i++) {
     if (pos >= 32) {
-      out[k++] = (uint32_t)lo;
-      lo = (lo >> 32) | (hi << 32);
-      hi >>= 32;
+      out[k++] = w0;
+      w0 = w1;
+      w1 = w2;
+      w2 = w3;
+      w3 = 0;
       pos -= 32;
     }
-    const unsigned long long add_lo = v[i] << pos;
-    const unsigned long long add_hi = pos ? (v[i] >> (64 - pos)) : 0ull;
-    lo += add_lo;
-    hi += add_hi + (lo < add_lo ? 1ull : 0ull);
+    const unsigned long long lo = v[i] << pos;
+    // v >> (64 - pos), 0 for pos = 0, without a select: (v >> 1) >> (63 - pos)
+    const unsigned hi = (unsigned)((unsigned)(v[i] >> 33) >> (31 - pos));
+    unsigned c0, c1, c2, c3;
+    w0 = __builtin_addc(w0, (unsigned)lo, 0u, &c0);
+    w1 = __builtin_addc(w1, (unsigned)(lo >> 32), c0, &c1);
+    w2 = __builtin_addc(w2, hi, c1, &c2);
+    w3 = __builtin_addc(w3, 0u, c2, &c3);
     pos += b;
   }
   for (; k < K; k++) {
-    out[k] = (uint32_t)lo;
-    lo = (lo >> 32) | (hi << 32);
-    hi >>= 32;
+    out[k] = w0;
+    w0 = w1;
+    w1 = w2;
+    w2 = w3;
+    w3 = 0;
   }
+}
+
+// Bit offset (j0 b) & 31 of a group's first digit j0 = t M + m0 (+ N): the
+// same for every row t of a tile, since 32 | M b in the passes that sum groups.
+__device__ __forceinline__ int group_pos0(const PassArgs &a, unsigned long long m0, int half) {
+  return (int)(((m0 + (half ? a.N : 0ull)) * (unsigned long long)a.bits) & 31ull);
 }
 
 // Group sums of the last inverse pass (group mode). After the final stage the
@@ -188,7 +206,7 @@ __device__ __forceinline__ void write_group_sums(const PassArgs &a, cd *lds, uns
   const int b = a.bits;
   const int K = a.ygK;
   for (int t = tid; t < L; t += TC::T) {
-    const int s0 = pad16(t * C);  // the row's C slots are contiguous
+    const int s0 = slot<C>(t, 0);  // the row's C slots are contiguous
     unsigned long long re[C], im[C];
 #pragma unroll
     for (int i = 0; i < C; i++) {
@@ -200,8 +218,7 @@ __device__ __forceinline__ void write_group_sums(const PassArgs &a, cd *lds, uns
 #pragma unroll
     for (int half = 0; half < 2; half++) {
       const unsigned long long j0 = (unsigned long long)t * a.M + m0 + (half ? a.N : 0ull);
-      const int pos0 = (int)((j0 * (unsigned long long)b) & 31ull);
-      group_sum_words<C>(half ? im : re, pos0, b, K, lw + half * K);
+      group_sum_words<C>(half ? im : re, group_pos0(a, m0, half), b, K, lw + half * K);
       uint4 *dst = reinterpret_cast<uint4 *>(a.ybuf + (j0 / C) * (unsigned long long)K);
       for (int q = 0; q < K / 4; q++)
         dst[q] = make_uint4(lw[half * K + 4 * q], lw[half * K + 4 * q + 1], lw[half * K + 4 * q + 2],
@@ -219,14 +236,13 @@ __device__ __forceinline__ void write_group_half(const PassArgs &a, long long *l
   const int b = a.bits;
   const int K = a.ygK;
   for (int t = tid; t < L; t += TC::T) {
-    const int s0 = pad16(t * C);  // the row's C slots are contiguous
+    const int s0 = slot<C>(t, 0);  // the row's C slots are contiguous
     unsigned long long v[C];
 #pragma unroll
     for (int i = 0; i < C; i++) v[i] = (unsigned long long)lc[s0 + i];
     uint32_t *lw = reinterpret_cast<uint32_t *>(lc + s0);
     const unsigned long long j0 = (unsigned long long)t * a.M + m0 + (half ? a.N : 0ull);
-    const int pos0 = (int)((j0 * (unsigned long long)b) & 31ull);
-    group_sum_words<C>(v, pos0, b, K, lw);
+    group_sum_words<C>(v, group_pos0(a, m0, half), b, K, lw);
     uint4 *dst = reinterpret_cast<uint4 *>(a.ybuf + (j0 / C) * (unsigned long long)K);
     for (int q = 0; q < K / 4; q++) dst[q] = make_uint4(lw[4 * q], lw[4 * q + 1], lw[4 * q + 2], lw[4 * q + 3]);
   }
@@ -276,19 +292,113 @@ __device__ __forceinline__ void split_transpose(double *ld, const cd (&x)[16], c
 #pragma unroll
     for (int t = 0; t < 16; t++) {
       const int row = to_blocks ? r0 + 16 * t : 16 * r0 + t;
-      ld[pad16(row * C + c)] = plane ? x[t].y : x[t].x;
+      ld[slot<C>(row, c)] = plane ? x[t].y : x[t].x;
     }
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < 16; t++) {
       const int row = to_blocks ? 16 * r0 + t : r0 + 16 * t;
       if (plane)
-        y[t].y = ld[pad16(row * C + c)];
+        y[t].y = ld[slot<C>(row, c)];
       else
-        y[t].x = ld[pad16(row * C + c)];
+        y[t].x = ld[slot<C>(row, c)];
     }
     __syncthreads();
   }
+}
+
+// Buffer resource over `words` 32-bit words at p: loads at byte offsets at or
+// beyond 4 * words return 0 (the resource's range check). The record count is
+// a 32-bit byte count; larger arrays are only ever read below 2^32 bytes.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t word_rsrc(const uint32_t *p, unsigned long long words) {
+  const unsigned bytes = words >= 0x3fffffffull ? 0xfffffffcu : (unsigned)(words * 4ull);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// Digits of the packing pass, read straight from the operand words. Lane
+// (r0, c) takes column m = m0 + c of rows r0 + h L/16 (h < 16) in both halves:
+// digit j = row M + m (+ N) starts at bit j b, that is at bit (m b) & 31 of
+// word row (M b / 32) + (m b >> 5) (+ N b / 32), as M b and N b are multiples
+// of 32. Words at or beyond al read as 0 through the range check. When b does
+// not divide 32 a digit may straddle two words; the second comes from a
+// resource based one word higher, so every load is one whole dword either in
+// or out of range.
+struct PackDigits {
+  __amdgpu_buffer_rsrc_t w0, w1, u0, u1;
+  unsigned off;   // byte offset of the lane's word in row r0
+  unsigned step;  // bytes between rows r0 + h L/16 and r0 + (h + 1) L/16
+  unsigned sh, bits;
+  bool upper;     // the upper half (digits N ..) holds words
+  bool one;       // b divides 32: a digit never straddles two words
+
+  template <int L>
+  __device__ __forceinline__ void init(const PassArgs &a, int r0, unsigned long long m) {
+    bits = (unsigned)a.bits;
+    const unsigned long long row_w = (a.M * bits) >> 5, half_w = (a.N * bits) >> 5, mb = m * bits;
+    const unsigned long long al = a.al;
+    w0 = word_rsrc(a.wa, al);
+    w1 = word_rsrc(a.wa + 1, al ? al - 1 : 0);
+    // an empty upper half gets empty resources: its loads return 0 without a
+    // memory access, and both halves' loads issue together
+    upper = half_w < al;
+    u0 = word_rsrc(upper ? a.wa + half_w : a.wa, upper ? al - half_w : 0);
+    u1 = word_rsrc(upper ? a.wa + half_w + 1 : a.wa, upper ? al - half_w - 1 : 0);
+    off = (unsigned)(((unsigned long long)r0 * row_w + (mb >> 5)) * 4ull);
+    step = (unsigned)(row_w * (unsigned long long)(L / 16) * 4ull);
+    sh = (unsigned)(mb & 31ull);
+    one = (bits & (bits - 1u)) == 0u;  // b <= 32 divides 32
+  }
+
+  // Lower-half digits into d, upper-half digits into e (0 when empty); every
+  // load is issued before the first is used.
+  __device__ __forceinline__ void load(unsigned (&d)[16], unsigned (&e)[16]) const {
+    unsigned lo[16], ulo[16];
+#pragma unroll
+    for (int h = 0; h < 16; h++) {
+      lo[h] = __builtin_amdgcn_raw_buffer_load_b32(w0, off + (unsigned)h * step, 0, 0);
+      ulo[h] = __builtin_amdgcn_raw_buffer_load_b32(u0, off + (unsigned)h * step, 0, 0);
+    }
+    if (one) {
+#pragma unroll
+      for (int h = 0; h < 16; h++) {
+        d[h] = __builtin_amdgcn_ubfe(lo[h], sh, bits);
+        e[h] = __builtin_amdgcn_ubfe(ulo[h], sh, bits);
+      }
+    } else {
+      unsigned hi[16], uhi[16];
+#pragma unroll
+      for (int h = 0; h < 16; h++) {
+        hi[h] = __builtin_amdgcn_raw_buffer_load_b32(w1, off + (unsigned)h * step, 0, 0);
+        uhi[h] = __builtin_amdgcn_raw_buffer_load_b32(u1, off + (unsigned)h * step, 0, 0);
+      }
+#pragma unroll
+      for (int h = 0; h < 16; h++) {
+        d[h] = __builtin_amdgcn_ubfe(__builtin_amdgcn_alignbit(hi[h], lo[h], sh), 0, bits);
+        e[h] = __builtin_amdgcn_ubfe(__builtin_amdgcn_alignbit(uhi[h], ulo[h], sh), 0, bits);
+      }
+    }
+  }
+};
+
+// Streaming loads and stores of the sixteen rows K r0 + S t (t < 16) of column
+// m0 + c, rows M points apart, for lane (r0, c) of a C-column tile: one lane
+// base address, plus the row step S t M, which is the same across the wave
+// (scalar registers), so each access costs one address addition and the
+// sixteen accesses hold no per-row address registers.
+template <int K, int S>
+__device__ __forceinline__ void load_rows16(const cd *base, cd (&v)[16], int r0, int c, unsigned long long m0,
+                                            unsigned long long M) {
+  const cd *lb = base + (unsigned long long)(K * r0) * M + m0 + c;
+#pragma unroll
+  for (int t = 0; t < 16; t++) v[t] = ld_nt(lb + (unsigned long long)(S * t) * M);
+}
+
+template <int K, int S>
+__device__ __forceinline__ void store_rows16(cd *base, const cd (&v)[16], int r0, int c, unsigned long long m0,
+                                             unsigned long long M) {
+  cd *lb = base + (unsigned long long)(K * r0) * M + m0 + c;
+#pragma unroll
+  for (int t = 0; t < 16; t++) st_nt(lb + (unsigned long long)(S * t) * M, v[t]);
 }
 
 template <int L, int C, bool PACK>
@@ -296,19 +406,17 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   using TC = Tile<L, C>;
   using ST = SplitTile<L, C>;
   bind_product(a);
-  // PACK stages the words under the tile (2 L segments of at most (20 C + 62) / 32 words)
-  constexpr int STAGE_BYTES = PACK ? 2 * L * ((C * 20 + 62) / 32) * 4 : 0;
   // the split transpose pays off where compute dominates (the packing pass);
   // the strided pass keeps the full tile (measured faster at fewer waves)
   constexpr bool SPLIT = ST::ON && PACK;
   constexpr int TILE_BYTES = SPLIT ? ST::BYTES : Tile<L, C>::LDS_ELEMS * 16;
-  __shared__ __align__(16) unsigned char lds_raw[TILE_BYTES > STAGE_BYTES ? TILE_BYTES : STAGE_BYTES];
+  __shared__ __align__(16) unsigned char lds_raw[TILE_BYTES];
   cd *lds = reinterpret_cast<cd *>(lds_raw);
   const int tid = threadIdx.x;
   const int tile = tile_of_block(a.ntiles);
   const unsigned long long tpb = a.M / C;
-  const unsigned long long ob = (unsigned long long)tile / tpb;
-  const unsigned long long m0 = ((unsigned long long)tile % tpb) * C;
+  const unsigned long long ob = (unsigned)tile / (unsigned)tpb;  // tpb < 2^31
+  const unsigned long long m0 = ((unsigned)tile % (unsigned)tpb) * (unsigned long long)C;
   cd *base = a.data + ob * a.S;
   constexpr int RB0 = TC::FIRSTB;
   constexpr int R0 = 1 << RB0;
@@ -320,67 +428,17 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   const int c = tid % C;
   const int r0 = tid / C;
   const unsigned long long m = m0 + c;
+  // PACK: digit pairs (a_j, a_{j+N}) of rows r0 + h L/16 and the right-angle
+  // weight Omega^{row M + m} = Omega^{r0 M + m} e^{2 pi i h / 64}
+  unsigned dre[PACK ? 16 : 1], dim_[PACK ? 16 : 1];
   cd wpack = make_double2(1.0, 0.0);
-  // right-angle weight Omega^{row M + m} = Omega^{r0 M + m} e^{2 pi i h / 64}
-  if (PACK) wpack = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
-  // PACK: the words under each of the 2L row segments (row t, half h: digits
-  // t M + m0 + h N + [0, C)) are staged in LDS with coalesced loads, then every
-  // thread slices its 32 digits out of LDS. M b and N b are multiples of 32
-  // (M >= 128, N >= 8192 in multi-pass plans), so a digit's bit offset inside
-  // its segment depends only on its column.
-  uint32_t dre[PACK && !SPLIT ? 16 : 1], dim_[PACK && !SPLIT ? 16 : 1];
-  bool upper = false;                                      // does the upper half hold digits?
-  unsigned p_sw = 0, p_mask = 0, p_kk = 0, p_sh = 0;      // digit slicing (PACK)
+  cd tb0[4];  // split tile: stage-0 level twiddles, loaded ahead of the digits
+  if constexpr (SPLIT) level_bases<4>(a.tl, r0, 1, tb0);
   if constexpr (PACK) {
-    uint32_t *lw = reinterpret_cast<uint32_t *>(lds);
-    const unsigned b = (unsigned)a.bits;
-    const unsigned SW = (C * b + 62) >> 5;                // words spanned by C digits
-    const unsigned inv = 0xffffffffu / SW + 1u;            // ceil(2^32 / SW)
-    const unsigned long long row_w = (a.M * b) >> 5;       // words per row of digits
-    const unsigned long long half_w = (a.N * b) >> 5;      // words per half
-    const unsigned long long w0 = (m0 * b) >> 5;
-    upper = half_w < a.al;
-    const unsigned n = (upper ? 2u : 1u) * L * SW;
-    // eight independent loads in flight per thread per batch
-    for (unsigned e0 = 0; e0 < n; e0 += 8u * TC::T) {
-      uint32_t r[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const unsigned e = e0 + (unsigned)u * TC::T + (unsigned)tid;
-        const unsigned seg = __umulhi(e, inv), k = e - seg * SW;
-        const unsigned long long wi =
-            (unsigned long long)(seg & (L - 1)) * row_w + w0 + (seg >= (unsigned)L ? half_w : 0ull) + k;
-        r[u] = (e < n && wi < a.al) ? a.wa[wi] : 0u;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const unsigned e = e0 + (unsigned)u * TC::T + (unsigned)tid;
-        if (e < n) lw[e] = r[u];
-      }
-    }
-    __syncthreads();
-    const unsigned mask = (1u << b) - 1u;
-    const unsigned off = (((unsigned)m0 * b) & 31u) + (unsigned)c * b;
-    const unsigned kk = off >> 5, sh = off & 31u;
-    p_sw = SW;
-    p_mask = mask;
-    p_kk = kk;
-    p_sh = sh;
-    if constexpr (!SPLIT) {
-#pragma unroll
-      for (int h = 0; h < 16; h++) {
-        const unsigned row = (unsigned)r0 + (unsigned)h * (L / 16);
-        const unsigned k0 = row * SW + kk;
-        dre[h] = (uint32_t)((((unsigned long long)lw[k0 + 1] << 32) | lw[k0]) >> sh) & mask;
-        if (upper) {
-          const unsigned k1 = k0 + (unsigned)L * SW;
-          dim_[h] = (uint32_t)((((unsigned long long)lw[k1 + 1] << 32) | lw[k1]) >> sh) & mask;
-        } else {
-          dim_[h] = 0u;
-        }
-      }
-      __syncthreads();
-    }
+    PackDigits pd;
+    pd.template init<L>(a, r0, m);
+    pd.load(dre, dim_);
+    wpack = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
   }
   if constexpr (SPLIT) {
     // PER0 = 1, SUB0 = 16: rows r0 + 16 t, then rows 16 r0 + t after the transpose
@@ -388,35 +446,23 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
 #pragma unroll
     for (int t = 0; t < 16; t++) {
       if constexpr (PACK) {
-        // the digits come straight from the staged words (no digit registers)
-        const uint32_t *lw = reinterpret_cast<const uint32_t *>(lds);
-        const unsigned k0 = ((unsigned)r0 + 16u * (unsigned)t) * p_sw + p_kk;
-        const cd w = cmul(wpack, w64(t));
-        const double re = (double)((uint32_t)((((unsigned long long)lw[k0 + 1] << 32) | lw[k0]) >> p_sh) & p_mask);
-        if (upper) {
-          const unsigned k1 = k0 + (unsigned)L * p_sw;
-          const double im = (double)((uint32_t)((((unsigned long long)lw[k1 + 1] << 32) | lw[k1]) >> p_sh) & p_mask);
-          x[t] = cmul(make_double2(re, im), w);
-        } else {
-          x[t] = make_double2(re * w.x, re * w.y);
-        }
+        // (a_j + i a_{j+N}) w (a_{j+N} = 0 throughout when the upper half is empty)
+        x[t] = cmul(make_double2((double)dre[t], (double)dim_[t]), cmul(wpack, w64(t)));
       } else {
         x[t] = ld_nt(&base[(unsigned long long)(r0 + 16 * t) * a.M + m]);
       }
     }
-    if constexpr (PACK) __syncthreads();  // the transpose reuses the staging area
-    dif_levels<4>(x, a.tl, r0, 1);
+    dif_levels<4>(x, tb0);
     split_transpose<C>(reinterpret_cast<double *>(lds_raw), x, y, r0, c, false);
     dif_levels_q0<4>(y);
     // row 16 r0 + t has frequency 16 brev4(t) + brev4(r0)
     Walk16 w;
     if (a.M > 1) w.template init_shared<C>(a.roots, a.tw_mul * m * brev((unsigned)r0, 4), a.tw_mul * m * 16ull);
+    if (a.M > 1) {
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
-      cd v = y[t];
-      if (a.M > 1) v = cmul(v, w.at(t));
-      st_nt(&base[(unsigned long long)(16 * r0 + t) * a.M + m], v);
+      for (int t = 0; t < 16; t++) y[t] = cmul(y[t], w.at(t));
     }
+    store_rows16<16, 1>(base, y, r0, c, m0, a.M);
     return;
   }
 #pragma unroll
@@ -428,17 +474,14 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
       unsigned long long row = (unsigned long long)(q + t * SUB0);
       if constexpr (PACK) {
         const int h = i + t * PER0;
-        const cd w = cmul(wpack, w64(h));
-        const double re = (double)dre[h];
-        // (a_j + i a_{j+N}) w, with a_{j+N} = 0 throughout when the upper half is empty
-        x[t] = upper ? cmul(make_double2(re, (double)dim_[h]), w) : make_double2(re * w.x, re * w.y);
+        x[t] = cmul(make_double2((double)dre[h], (double)dim_[h]), cmul(wpack, w64(h)));
       } else {
         x[t] = ld_nt(&base[row * a.M + m]);
       }
     }
     dif_levels<RB0>(x, a.tl, q, 1);
 #pragma unroll
-    for (int t = 0; t < R0; t++) lds[pad16((q + t * SUB0) * C + c)] = x[t];
+    for (int t = 0; t < R0; t++) lds[slot<C>(q + t * SUB0, c)] = x[t];
   }
   __syncthreads();
   forward_rest<L, C>(lds, a.tl, tid);
@@ -460,7 +503,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
 #pragma unroll
     for (int h = 0; h < 16; h++) {
       const int row = r0 + h * (L / 16);
-      cd v = lds[pad16(row * C + c)];
+      cd v = lds[slot<C>(row, c)];
       if (a.M > 1) v = cmul(v, w.at(h));
       st_nt(&base[(unsigned long long)row * a.M + m], v);
     }
@@ -484,8 +527,8 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
   const int tid = threadIdx.x;
   const int tile = tile_of_block(a.ntiles);
   const unsigned long long tpb = a.M / C;
-  const unsigned long long ob = (unsigned long long)tile / tpb;
-  const unsigned long long m0 = ((unsigned long long)tile % tpb) * C;
+  const unsigned long long ob = (unsigned)tile / (unsigned)tpb;  // tpb < 2^31
+  const unsigned long long m0 = ((unsigned)tile % (unsigned)tpb) * (unsigned long long)C;
   cd *base = a.data + ob * a.S;
   static_assert(!UNPACK || !TC::THREE, "unpacking pass is a power of two");
   const int c = tid % C;
@@ -494,9 +537,9 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
   if constexpr (ST::ON) {
     // rows 16 r0 + t (frequency 16 brev4(t) + brev4(r0)) with the conjugate
     // pass twiddle, stage 1 in registers, transpose to rows r0 + 16 t, stage 0
-    cd x[16], y[16];
-#pragma unroll
-    for (int t = 0; t < 16; t++) y[t] = ld_nt(&base[(unsigned long long)(16 * r0 + t) * a.M + m]);
+    cd x[16], y[16], tb0[4];
+    level_bases<4>(a.tl, r0, 1, tb0);  // stage-0 level twiddles, ahead of the data
+    load_rows16<16, 1>(base, y, r0, c, m0, a.M);
     Walk16 w;
     if (a.M > 1) {
       w.template init_shared<C>(a.roots, a.tw_mul * m * brev((unsigned)r0, 4), a.tw_mul * m * 16ull);
@@ -506,10 +549,9 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
     dit_levels_q0<4>(y);
     double *ld = reinterpret_cast<double *>(lds_raw);
     split_transpose<C>(ld, y, x, r0, c, true);
-    dit_levels<4>(x, a.tl, r0, 1);
+    dit_levels<4>(x, tb0);
     if constexpr (!UNPACK) {
-#pragma unroll
-      for (int t = 0; t < 16; t++) st_nt(&base[(unsigned long long)(r0 + 16 * t) * a.M + m], x[t]);
+      store_rows16<1, 16>(base, x, r0, c, m0, a.M);
     } else {
       double worst = 0.0;
       const cd wun = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
@@ -521,14 +563,14 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
 #pragma unroll
         for (int t = 0; t < 16; t++) {
           const cd f = cmulc(x[t], cmul(wun, w64(t)));
-          lc[pad16((r0 + 16 * t) * C + c)] = round_coef(f.x * a.inv_n, true, worst);
+          lc[slot<C>(r0 + 16 * t, c)] = round_coef(f.x * a.inv_n, true, worst);
           ci[t] = round_coef(f.y * a.inv_n, true, worst);
         }
         __syncthreads();
         write_group_half<L, C>(a, lc, m0, tid, 0);
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < 16; t++) lc[pad16((r0 + 16 * t) * C + c)] = ci[t];
+        for (int t = 0; t < 16; t++) lc[slot<C>(r0 + 16 * t, c)] = ci[t];
         __syncthreads();
         write_group_half<L, C>(a, lc, m0, tid, 1);
       } else {
@@ -555,14 +597,13 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
     // all sixteen loads in flight first, then the conjugate pass twiddle
     // (rows r0 + h * L/16 as in k_fwd) on the way into LDS
     cd xs[16];
-#pragma unroll
-    for (int h = 0; h < 16; h++) xs[h] = ld_nt(&base[(unsigned long long)(r0 + h * (L / 16)) * a.M + m]);
+    load_rows16<1, L / 16>(base, xs, r0, c, m0, a.M);
     Walk16 w;
     if (a.M > 1) w.template init_shared<C>(a.roots, a.tw_mul * m * 16ull * brev((unsigned)r0, TC::LOG - 4), a.tw_mul * m);
 #pragma unroll
     for (int h = 0; h < 16; h++) {
       const int row = r0 + h * (L / 16);
-      lds[pad16(row * C + c)] = (a.M > 1) ? cmulc(xs[h], w.at(h)) : xs[h];
+      lds[slot<C>(row, c)] = (a.M > 1) ? cmulc(xs[h], w.at(h)) : xs[h];
     }
   }
   __syncthreads();
@@ -580,7 +621,7 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
     int q = r0 + i * (TC::T / C);
     cd x[R0];
 #pragma unroll
-    for (int t = 0; t < R0; t++) x[t] = lds[pad16((q + t * SUB0) * C + c)];
+    for (int t = 0; t < R0; t++) x[t] = lds[slot<C>(q + t * SUB0, c)];
     dit_levels<RB0>(x, a.tl, q, 1);
 #pragma unroll
     for (int t = 0; t < R0; t++) {
@@ -593,7 +634,7 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
           // group mode implies coefficients below 2^51
           const long long cr = round_coef(f.x * a.inv_n, true, worst);
           const long long ci = round_coef(f.y * a.inv_n, true, worst);
-          reinterpret_cast<longlong2 *>(lds)[pad16((q + t * SUB0) * C + c)] = make_longlong2(cr, ci);
+          reinterpret_cast<longlong2 *>(lds)[slot<C>(q + t * SUB0, c)] = make_longlong2(cr, ci);
         } else {
           worst = fmax(worst, round_store(a, f, row, m, L));
         }
@@ -640,7 +681,7 @@ __device__ __forceinline__ void mid_forward(const PassArgs &a, cd *lds, const cd
       for (int t = 0; t < R0; t++) xr[i * R0 + t] = x[t];
     } else {
 #pragma unroll
-      for (int t = 0; t < R0; t++) lds[pad16(q + t * SUB0)] = x[t];
+      for (int t = 0; t < R0; t++) lds[slot_step<SUB0>(q, t)] = x[t];
     }
   }
   if (TC::NST2 == 1 && !TC::THREE) return;
@@ -675,7 +716,7 @@ __device__ __forceinline__ void mid_forward(const PassArgs &a, cd *lds, const cd
       int blk = tid + i * TC::T;
       cd x[R];
 #pragma unroll
-      for (int t = 0; t < R; t++) x[t] = lds[pad16(blk * R + t)];
+      for (int t = 0; t < R; t++) x[t] = lds[slot<R>(blk, t)];
       dif_levels_q0<RBL>(x);
 #pragma unroll
       for (int t = 0; t < R; t++) xr[i * R + t] = x[t];
@@ -731,7 +772,7 @@ __device__ __forceinline__ void mid_pipeline(const PassArgs &a, cd *lds, unsigne
         for (int t = 0; t < R; t++) x[t] = xa[i * R + t];
         dit_levels_q0<RBL>(x);
 #pragma unroll
-        for (int t = 0; t < R; t++) lds[pad16(blk * R + t)] = x[t];
+        for (int t = 0; t < R; t++) lds[slot<R>(blk, t)] = x[t];
       }
     }
     __syncthreads();
@@ -755,7 +796,7 @@ __device__ __forceinline__ void mid_pipeline(const PassArgs &a, cd *lds, unsigne
       for (int t = 0; t < R0; t++) x[t] = xa[i * R0 + t];
     } else {
 #pragma unroll
-      for (int t = 0; t < R0; t++) x[t] = lds[pad16(q + t * SUB0)];
+      for (int t = 0; t < R0; t++) x[t] = lds[slot_step<SUB0>(q, t)];
     }
     dit_levels<RB0>(x, a.tl, q, 1);
 #pragma unroll

@@ -1094,10 +1094,10 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
 // last) with a running group index, forms P_w, v_w = lo(P_w) + hi(P_{w-1})
 // and the carry-function scan of k_carry_local.
 constexpr int GCT = 256;
-constexpr int GWPT = 16;
+constexpr int GWPT = 8;  // 2048-word blocks: 17 KB of staged groups, nine blocks per CU
 template <int GCOV>  // group sums that can cover one word (<= CARRY_GCOV)
 static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
-  extern __shared__ uint32_t gl[];
+  extern __shared__ uint32_t gl_w[];
   __shared__ unsigned wave_fn[GCT / 64];
   const int tid = threadIdx.x;
   const long long W0 = (long long)blockIdx.x * GCT * GWPT;
@@ -1108,7 +1108,7 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
   if (g_first < 0) g_first = 0;
   long long g_last = div_small(32 * (W0 + (long long)GCT * GWPT - 1) + 31, gb, inv_gb);
   if (g_last >= ng) g_last = ng - 1;
-  // LDS word i lives at i + i / 32: threads read words 16 apart, and the pad
+  // LDS word i lives at i + i / 32: threads read words GWPT apart, and the pad
   // keeps those reads on distinct banks
   {
     const long long nvec = g_last >= g_first ? (g_last - g_first + 1) * a.gKs / 4 : 0;
@@ -1125,46 +1125,59 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
         const long long t = t0 + u * GCT + tid;
         if (t < nvec) {
           const unsigned i = (unsigned)(4 * t), pi = i + (i >> 5);  // 4 words never straddle a pad
-          gl[pi] = r[u].x;
-          gl[pi + 1] = r[u].y;
-          gl[pi + 2] = r[u].z;
-          gl[pi + 3] = r[u].w;
+          gl_w[pi] = r[u].x;
+          gl_w[pi + 1] = r[u].y;
+          gl_w[pi + 2] = r[u].z;
+          gl_w[pi + 3] = r[u].w;
         }
       }
     }
   }
   __syncthreads();
   const long long wfirst = W0 + (long long)tid * GWPT;
-  // running group: g = last group with W_g <= w; Wt[t] = W_{g - t}
-  long long w = wfirst - 1;
-  long long g = w < 0 ? 0 : div_small(32 * w + 31, gb, inv_gb);
-  if (g >= ng) g = ng - 1;
-  long long Wt[GCOV];
+  // Block-relative 32-bit arithmetic: staged group gl (group g_first + gl)
+  // starts at bit B0 + gl gb, B0 = g_first gb, i.e. in block word
+  // D + ((r0 + gl gb) >> 5), r0 = B0 & 31, D = (B0 >> 5) - W0 (<= 0).
+  const unsigned long long B0 = (unsigned long long)g_first * (unsigned long long)gb;
+  const int r0 = (int)(B0 & 31ull);
+  const int D = (int)((long long)(B0 >> 5) - W0);
+  const int ngl = (int)(g_last - g_first + 1);
+  // words below lim (relative) exist: w < CL
+  const long long lim64 = (long long)a.CL - W0;
+  const int lim = lim64 > (1 << 30) ? (1 << 30) : (int)lim64;
+  const int wmin = W0 == 0 ? 0 : -1;  // word W0 - 1 exists unless W0 = 0
+  // running group: gl = last staged group starting at or before word wl (-1: none)
+  int wl = tid * GWPT - 1;
+  const int num = 32 * (wl - D) + 31 - r0;
+  int gl = num < 0 ? -1 : (int)((unsigned)num / (unsigned)gb);
+  if (gl > ngl - 1) gl = ngl - 1;
+  int Wt[GCOV];  // Wt[t]: first block word of group gl - t
 #pragma unroll
-  for (int t = 0; t < GCOV; t++) Wt[t] = ((g - t) * gb) >> 5;
+  for (int t = 0; t < GCOV; t++) Wt[t] = ((r0 + (gl - t) * gb) >> 5) + D;
+  int wnext = ((r0 + (gl + 1) * gb) >> 5) + D;  // first word of group gl + 1
   unsigned long long P[GWPT + 1];
 #pragma unroll
-  for (int k = 0; k <= GWPT; k++) {
-    w = wfirst - 1 + k;
-    if (k > 0 && g + 1 < ng && (((g + 1) * gb) >> 5) <= w) {
-      g++;
+  for (int k = 0; k <= GWPT; k++, wl++) {
+    if (k > 0 && gl + 1 < ngl && wnext <= wl) {
+      gl++;
 #pragma unroll
       for (int t = GCOV - 1; t > 0; t--) Wt[t] = Wt[t - 1];
-      Wt[0] = (g * gb) >> 5;
+      Wt[0] = wnext;
+      wnext = ((r0 + (gl + 1) * gb) >> 5) + D;
     }
+    // branch-free: out-of-range terms read slot 0 and count as 0
     unsigned long long p = 0;
-    if (w >= 0 && w < (long long)a.CL) {
 #pragma unroll
-      for (int t = 0; t < GCOV; t++) {
-        const long long off = w - Wt[t];
-        if (g - t >= g_first && off < a.gK) {
-          const unsigned i = (unsigned)((g - t - g_first) * a.gKs + off);
-          p += gl[i + (i >> 5)];
-        }
-      }
-      if (a.add && w < (long long)a.addl) p += a.add[w];
+    for (int t = 0; t < GCOV; t++) {
+      const int off = wl - Wt[t];
+      const bool ok = gl - t >= 0 && off < a.gK;
+      const unsigned i = ok ? (unsigned)((gl - t) * a.gKs + off) : 0u;
+      const unsigned x = gl_w[i + (i >> 5)];
+      p += ok ? x : 0u;
     }
-    P[k] = p;
+    const bool here = wl >= wmin && wl < lim;
+    if (a.add && here && W0 + wl < (long long)a.addl) p += a.add[W0 + wl];
+    P[k] = here ? p : 0ull;
   }
   if (wfirst == 0) P[0] = 0;
   unsigned long long v[GWPT];

@@ -513,12 +513,17 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
 // ------------------------------------------------------------ inverse pass
 
 // Mirror of k_fwd: staged load with the conjugate pass twiddle, DIT stages in
-// LDS, stage 0 stores straight to global, or (UNPACK) rounds to coefficients.
+// LDS, stage 0 stores straight to global, or rounds to coefficients: MODE 0
+// stores points, MODE 1 writes the 2N coefficients, MODE 2 their group sums
+// (a.ybuf). The two unpacking modes are separate kernels, so the group mode
+// is not held to the registers of the coefficient stores.
 // The unpacking pass of a split tile is held to 168 VGPRs (three waves per
 // SIMD; measured faster than two waves without the two spilled registers).
-template <int L, int C, bool UNPACK>
+template <int L, int C, int MODE>
 __global__ __launch_bounds__((Tile<L, C>::T))
-__attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3 : 1, 8))) void k_inv(PassArgs a) {
+__attribute__((amdgpu_waves_per_eu((MODE == 1 && SplitTile<L, C>::ON && C >= 8) ? 3 : 1, 8))) void k_inv(PassArgs a) {
+  constexpr bool UNPACK = MODE != 0;
+  constexpr bool GROUPS = MODE == 2;
   using TC = Tile<L, C>;
   using ST = SplitTile<L, C>;
   bind_product(a);
@@ -538,7 +543,6 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
     // rows 16 r0 + t (frequency 16 brev4(t) + brev4(r0)) with the conjugate
     // pass twiddle, stage 1 in registers, transpose to rows r0 + 16 t, stage 0
     cd x[16], y[16], tb0[4];
-    level_bases<4>(a.tl, r0, 1, tb0);  // stage-0 level twiddles, ahead of the data
     load_rows16<16, 1>(base, y, r0, c, m0, a.M);
     Walk16 w;
     if (a.M > 1) {
@@ -547,6 +551,8 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
       for (int t = 0; t < 16; t++) y[t] = cmulc(y[t], w.at(t));
     }
     dit_levels_q0<4>(y);
+    // stage-0 level twiddles, loaded while the transpose runs
+    level_bases<4>(a.tl, r0, 1, tb0);
     double *ld = reinterpret_cast<double *>(lds_raw);
     split_transpose<C>(ld, y, x, r0, c, true);
     dit_levels<4>(x, tb0);
@@ -555,7 +561,7 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
     } else {
       double worst = 0.0;
       const cd wun = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
-      if (a.ybuf) {
+      if constexpr (GROUPS) {
         // group mode (coefficients below 2^51): the real coefficients, then
         // the imaginary ones, each summed per row from LDS
         long long *lc = reinterpret_cast<long long *>(lds_raw);
@@ -615,7 +621,7 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
   double worst = 0.0;
   cd wun = make_double2(1.0, 0.0);
   if (UNPACK) wun = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
-  const bool groups = UNPACK && a.ybuf;
+  constexpr bool groups = GROUPS;
 #pragma unroll
   for (int i = 0; i < PER0; i++) {
     int q = r0 + i * (TC::T / C);
@@ -644,7 +650,7 @@ __attribute__((amdgpu_waves_per_eu((UNPACK && SplitTile<L, C>::ON && C >= 8) ? 3
     }
   }
   if constexpr (UNPACK) {
-    if (groups) {
+    if constexpr (groups) {
       __syncthreads();
       write_group_sums<L, C>(a, lds, m0, tid);
     }

@@ -24,16 +24,23 @@ int launch_fwd(bool pack, int grid, int gy, hipStream_t s, const PassArgs &a) {
   return 0;
 }
 
+// Unpacking into group sums (a.ybuf) exists for 8- and 16-column tiles.
 template <int L, int C>
 int launch_inv(bool unpack, int grid, int gy, hipStream_t s, const PassArgs &a) {
   if constexpr (Len<L>::THREE) {
     if (unpack) return -2;
-    BCG_LAUNCH2((k_inv<L, C, false>), grid, gy, (Tile<L, C>::T), s, a);
+    BCG_LAUNCH2((k_inv<L, C, 0>), grid, gy, (Tile<L, C>::T), s, a);
   } else {
-    if (unpack)
-      BCG_LAUNCH2((k_inv<L, C, true>), grid, gy, (Tile<L, C>::T), s, a);
-    else
-      BCG_LAUNCH2((k_inv<L, C, false>), grid, gy, (Tile<L, C>::T), s, a);
+    if (unpack && a.ybuf) {
+      if constexpr (C == 8 || C == 16)
+        BCG_LAUNCH2((k_inv<L, C, 2>), grid, gy, (Tile<L, C>::T), s, a);
+      else
+        return -2;
+    } else if (unpack) {
+      BCG_LAUNCH2((k_inv<L, C, 1>), grid, gy, (Tile<L, C>::T), s, a);
+    } else {
+      BCG_LAUNCH2((k_inv<L, C, 0>), grid, gy, (Tile<L, C>::T), s, a);
+    }
   }
   return 0;
 }

@@ -19,7 +19,7 @@ fwd_fn pick_inv_pow2(int L, int C) {
 // first launch of one of its kernels, inside a caller's first product.
 int preload_inv_pow2() {
   hipFuncAttributes fa;
-  return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&(k_inv<256, 16, false>))) == hipSuccess ? 0 : -1;
+  return hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&(k_inv<256, 16, 0>))) == hipSuccess ? 0 : -1;
 }
 
 }  // namespace bcg

@@ -807,6 +807,7 @@ int carry_of_plan(const Plan &p, uint32_t *out, unsigned long long CL, hipStream
   c.addl = addl;
   c.ybuf = (const uint32_t *)g.bufB.p;
   c.gC = p.C[0];
+  c.ng = (long long)(c.ncoef / (unsigned long long)c.gC);
   c.gK = K;
   c.gKs = group_stride(K);
   // LDS: the groups under one block's GCT * GWPT words

@@ -882,6 +882,7 @@ struct CarryArgs {
   unsigned lds_slots;        // int64 slots of dynamic LDS per block (coefficient window)
   const uint32_t *ybuf;      // group mode: group sums instead of coefficients (ncoef digits in all)
   int gC, gK, gKs;           // digits per group, words per group sum, group stride (words)
+  long long ng;              // groups: ncoef / gC
   unsigned wpb;              // words per carry block (k_carry_fix)
 };
 
@@ -1109,7 +1110,7 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
   const long long W0 = (long long)blockIdx.x * GCT * GWPT;
   const int gb = a.gC * a.bits;  // bits per group, >= 32
   const double inv_gb = 1.0 / (double)gb;
-  const long long ng = (long long)(a.ncoef / (unsigned long long)a.gC);
+  const long long ng = a.ng;
   long long g_first = W0 == 0 ? 0 : div_small(32 * (W0 - 1) + 31, gb, inv_gb) - (CARRY_GCOV - 1);
   if (g_first < 0) g_first = 0;
   long long g_last = div_small(32 * (W0 + (long long)GCT * GWPT - 1) + 31, gb, inv_gb);
@@ -1117,18 +1118,18 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
   // LDS word i lives at i + i / 32: threads read words GWPT apart, and the pad
   // keeps those reads on distinct banks
   {
-    const long long nvec = g_last >= g_first ? (g_last - g_first + 1) * a.gKs / 4 : 0;
+    const int nvec = g_last >= g_first ? (int)((g_last - g_first + 1) * a.gKs / 4) : 0;
     const uint4 *src = reinterpret_cast<const uint4 *>(a.ybuf + g_first * a.gKs);
-    for (long long t0 = 0; t0 < nvec; t0 += 8 * GCT) {
+    for (int t0 = 0; t0 < nvec; t0 += 8 * GCT) {
       uint4 r[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-        const long long t = t0 + u * GCT + tid;
+        const int t = t0 + u * GCT + tid;
         r[u] = t < nvec ? src[t] : make_uint4(0u, 0u, 0u, 0u);
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-        const long long t = t0 + u * GCT + tid;
+        const int t = t0 + u * GCT + tid;
         if (t < nvec) {
           const unsigned i = (unsigned)(4 * t), pi = i + (i >> 5);  // 4 words never straddle a pad
           gl_w[pi] = r[u].x;
@@ -1152,38 +1153,76 @@ static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
   const long long lim64 = (long long)a.CL - W0;
   const int lim = lim64 > (1 << 30) ? (1 << 30) : (int)lim64;
   const int wmin = W0 == 0 ? 0 : -1;  // word W0 - 1 exists unless W0 = 0
-  // running group: gl = last staged group starting at or before word wl (-1: none)
-  int wl = tid * GWPT - 1;
-  const int num = 32 * (wl - D) + 31 - r0;
-  int gl = num < 0 ? -1 : (int)((unsigned)num / (unsigned)gb);
-  if (gl > ngl - 1) gl = ngl - 1;
-  int Wt[GCOV];  // Wt[t]: first block word of group gl - t
-#pragma unroll
-  for (int t = 0; t < GCOV; t++) Wt[t] = ((r0 + (gl - t) * gb) >> 5) + D;
-  int wnext = ((r0 + (gl + 1) * gb) >> 5) + D;  // first word of group gl + 1
   unsigned long long P[GWPT + 1];
+  if ((gb & 31) == 0) {
+    // Word-aligned group starts (G b a multiple of 32, e.g. b even): group gl
+    // starts at block word D + gl Q, so word wl = D + gh Q + o takes word
+    // o + t Q of group gh - t, at LDS index ih - t (gKs - Q), ih = gh gKs + o;
+    // all advance by one per word. o < Q < K, so term 0 only needs gh >= 0,
+    // and gh < ngl for every word of the block.
+    const int Q = gb >> 5, dg = a.gKs - Q;
+    int wl = tid * GWPT - 1;
+    const int x0 = wl - D;  // >= -1
+    int gh = x0 < 0 ? -1 : (int)((unsigned)x0 / (unsigned)Q);
+    int o = x0 - gh * Q;
+    int ih = gh * a.gKs + o;
 #pragma unroll
-  for (int k = 0; k <= GWPT; k++, wl++) {
-    if (k > 0 && gl + 1 < ngl && wnext <= wl) {
-      gl++;
+    for (int k = 0; k <= GWPT; k++, wl++) {
+      if (k > 0) {
+        const bool wrap = o + 1 == Q;
+        o = wrap ? 0 : o + 1;
+        gh += wrap ? 1 : 0;
+        ih += wrap ? 1 + dg : 1;
+      }
+      const unsigned i0 = gh >= 0 ? (unsigned)ih : 0u;
+      unsigned long long p = gh >= 0 ? gl_w[i0 + (i0 >> 5)] : 0u;
 #pragma unroll
-      for (int t = GCOV - 1; t > 0; t--) Wt[t] = Wt[t - 1];
-      Wt[0] = wnext;
-      wnext = ((r0 + (gl + 1) * gb) >> 5) + D;
+      for (int t = 1; t < GCOV; t++) {
+        const bool ok = gh >= t && o + t * Q < a.gK;
+        const unsigned i = ok ? (unsigned)(ih - t * dg) : 0u;
+        const unsigned x = gl_w[i + (i >> 5)];
+        p += ok ? x : 0u;
+      }
+      P[k] = (wl >= wmin && wl < lim) ? p : 0ull;
     }
-    // branch-free: out-of-range terms read slot 0 and count as 0
-    unsigned long long p = 0;
+  } else {
+    // running group: gl = last staged group starting at or before word wl (-1: none)
+    int wl = tid * GWPT - 1;
+    const int num = 32 * (wl - D) + 31 - r0;
+    int gl = num < 0 ? -1 : (int)((unsigned)num / (unsigned)gb);
+    if (gl > ngl - 1) gl = ngl - 1;
+    int Wt[GCOV];  // Wt[t]: first block word of group gl - t
 #pragma unroll
-    for (int t = 0; t < GCOV; t++) {
-      const int off = wl - Wt[t];
-      const bool ok = gl - t >= 0 && off < a.gK;
-      const unsigned i = ok ? (unsigned)((gl - t) * a.gKs + off) : 0u;
-      const unsigned x = gl_w[i + (i >> 5)];
-      p += ok ? x : 0u;
+    for (int t = 0; t < GCOV; t++) Wt[t] = ((r0 + (gl - t) * gb) >> 5) + D;
+    int wnext = ((r0 + (gl + 1) * gb) >> 5) + D;  // first word of group gl + 1
+#pragma unroll
+    for (int k = 0; k <= GWPT; k++, wl++) {
+      if (k > 0 && gl + 1 < ngl && wnext <= wl) {
+        gl++;
+#pragma unroll
+        for (int t = GCOV - 1; t > 0; t--) Wt[t] = Wt[t - 1];
+        Wt[0] = wnext;
+        wnext = ((r0 + (gl + 1) * gb) >> 5) + D;
+      }
+      // branch-free: out-of-range terms read slot 0 and count as 0
+      unsigned long long p = 0;
+#pragma unroll
+      for (int t = 0; t < GCOV; t++) {
+        const int off = wl - Wt[t];
+        const bool ok = gl - t >= 0 && off < a.gK;
+        const unsigned i = ok ? (unsigned)((gl - t) * a.gKs + off) : 0u;
+        const unsigned x = gl_w[i + (i >> 5)];
+        p += ok ? x : 0u;
+      }
+      P[k] = (wl >= wmin && wl < lim) ? p : 0ull;
     }
-    const bool here = wl >= wmin && wl < lim;
-    if (a.add && here && W0 + wl < (long long)a.addl) p += a.add[W0 + wl];
-    P[k] = here ? p : 0ull;
+  }
+  if (a.add) {  // the addend's words, where they exist
+#pragma unroll
+    for (int k = 0; k <= GWPT; k++) {
+      const long long w = wfirst - 1 + k;
+      if (w >= 0 && w < (long long)a.CL && w < (long long)a.addl) P[k] += a.add[w];
+    }
   }
   if (wfirst == 0) P[0] = 0;
   unsigned long long v[GWPT];

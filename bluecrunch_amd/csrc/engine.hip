@@ -1863,6 +1863,9 @@ int batch_plan(BatchPlan &bp, uint32_t *C, const uint32_t *A, const uint32_t *B,
   };
   std::map<int, std::vector<size_t>> classes;
   std::map<unsigned long long, std::vector<size_t>> mclasses;
+  // the products of one tree level mostly share their lengths: plan each
+  // length pair once
+  std::map<std::pair<size_t, size_t>, Plan> plan_of;
   for (size_t i = 0; i < count; i++) {
     const uint32_t *pa = A + a_off[i];
     const uint32_t *pb = B + b_off[i];
@@ -1881,7 +1884,13 @@ int batch_plan(BatchPlan &bp, uint32_t *C, const uint32_t *A, const uint32_t *B,
         bp.tiny.push_back(desc_of(i, nullptr));
       continue;
     }
-    if (int rc = make_plan(al[i], bl[i], 0, bp.plans[i], true)) return rc;
+    auto pit = plan_of.find({al[i], bl[i]});
+    if (pit == plan_of.end()) {
+      Plan pl;
+      if (int rc = make_plan(al[i], bl[i], 0, pl, true)) return rc;
+      pit = plan_of.emplace(std::make_pair(al[i], bl[i]), pl).first;
+    }
+    bp.plans[i] = pit->second;
     if (alias)
       bp.single.push_back(i);
     else if (bp.plans[i].P == 1)

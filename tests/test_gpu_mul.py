@@ -187,6 +187,30 @@ def test_forced_digit_sizes(bc, bits):
     assert np.array_equal(dC.to_host(), exact_product(a, b))
 
 
+@pytest.mark.parametrize("bits", [5, 6, 7, 9, 11, 13])
+def test_forced_digit_sizes_ragged(bc, bits):
+    # odd lengths leave the upper half of the packing pass partly filled and,
+    # with odd digit sizes, digits straddle words (two-word loads) up to the
+    # operands' last word, and group starts are not word-aligned in the carry
+    a, b = operands(100 + bits, 20011, 19993)
+    dA, dB, dC = bc.DeviceWords.from_host(a), bc.DeviceWords.from_host(b), bc.DeviceWords(40004)
+    info = bc.mul_dev(dC.ptr, 40004, dA.ptr, 20011, dB.ptr, 19993, bits=bits)
+    assert info["bits"] <= bits and info["passes"] > 1
+    assert np.array_equal(dC.to_host(), exact_product(a, b, 40004))
+
+
+@pytest.mark.parametrize("words", [20011, 300007])
+def test_muladd_group_carry(bc, words):
+    # a*b + d through the group-sum carry (multi-pass sizes): the addend's
+    # words enter every carry block they reach, a carry runs out of its top
+    a, b = operands(words, words, words - 17)
+    d = np.full(words + 5, 0xffffffff, dtype=np.uint32)
+    c, info = bc.muladd(a, b, d)
+    want = words_to_int(a) * words_to_int(b) + words_to_int(d)
+    assert words_to_int(c) == want % (1 << (32 * c.size))
+    assert info["margin"] < 0.25
+
+
 @pytest.mark.parametrize("words,stride", [(9000, 1), (100000, 1), (700000, 5)])
 def test_every_pass_split_is_exact(bc, words, stride):
     # every factorisation the autotuner may pick must give the same exact

@@ -32,7 +32,7 @@ EXPORTS = {
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
         "bcg_plan_force", "bcg_plan_current", "bcg_plan_digit_bias", "bcg_autotune_classes", "bcg_plan_force_batch", "bcg_plan_current_batch", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_test_fail_alloc", "bcg_bf_set", "bcg_bf_get", "bcg_bf_free", "bcg_bf_add", "bcg_bf_sub", "bcg_bf_mul", "bcg_bf_rcp", "bcg_bf_div", "bcg_prof_enable", "bcg_prof_select", "bcg_prof_read",
-        "bcg_prof_reset", "bcg_stats_reset", "bcg_stats_get",
+        "bcg_prof_reset", "bcg_stats_reset", "bcg_stats_get", "bcg_defer_begin", "bcg_defer_end",
     ],
     "fft.h": [
         "twiddle_table", "twiddle_table_size", "fft_ensure_table", "table_select", "fft_free_table",
@@ -141,6 +141,8 @@ def lib():
     L.bcg_prof_select.argtypes = [ctypes.c_uint]
     L.bcg_prof_read.argtypes = [i, ctypes.POINTER(ctypes.c_long), _f64p]
     L.bcg_stats_get.argtypes = [ctypes.POINTER(Stats)]
+    L.bcg_defer_begin.argtypes = []
+    L.bcg_defer_end.argtypes = [ctypes.POINTER(ctypes.c_double)]
     # reference C API
     L.fft_length.argtypes = [_sz]; L.fft_length.restype = _sz
     L.table_select.argtypes = [_sz]; L.table_select.restype = _sz
@@ -471,6 +473,20 @@ def plan_candidates(n):
 
 def plan_force(n, index):
     check(lib().bcg_plan_force(n, index), "plan_force")
+
+
+def defer_begin():
+    """Open a deferral scope: this thread's products stop reading their
+    margins back one by one (bcg_defer_begin)."""
+    check(lib().bcg_defer_begin(), "defer_begin")
+
+
+def defer_end():
+    """Close the scope; returns its worst rounding margin (results of the
+    scope are exact only below 0.25)."""
+    w = ctypes.c_double()
+    check(lib().bcg_defer_end(ctypes.byref(w)), "defer_end")
+    return w.value
 
 
 def plan_digit_bias(extra_bits):

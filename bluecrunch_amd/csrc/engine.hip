@@ -838,7 +838,32 @@ int zero_margin(hipStream_t s) {
   return 0;
 }
 
+// Deferred margin checks (bcg_defer_begin / bcg_defer_end): the products of
+// the thread that opened the scope each write their margin slots into a set
+// of their own and are not read back or re-run one by one (no host
+// synchronisation per product); the sets are read together when they run
+// out and at the end of the scope, which returns the worst margin. Products
+// whose output aliases an input, and all products of other threads, check
+// at once as usual.
+constexpr int DEFER_SETS = 64;
+constexpr size_t MARGIN_SET_BYTES = sizeof(unsigned int) * MARGIN_SLOTS * MARGIN_STRIDE;
+struct DeferScope {
+  bool open = false;
+  bool in_product = false;  // a deferred product is running: its margin is not read
+  std::thread::id owner;
+  DevBuf sets;
+  int n = 0;  // sets in use
+  unsigned long long points[DEFER_SETS];
+  int bits[DEFER_SETS];
+  double worst = 0.0;
+};
+DeferScope g_defer;
+
 int read_margin(hipStream_t s, double *m) {
+  if (g_defer.in_product) {  // read with the scope's other sets later
+    *m = 0.0;
+    return 0;
+  }
   unsigned int slots[MARGIN_SLOTS * MARGIN_STRIDE];
   if (int rc = d2h(slots, g.margin.p, sizeof slots, s)) return rc;
   unsigned int bits = 0;
@@ -1205,8 +1230,57 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
   return 0;
 }
 
+// Reads the margin sets of the open deferral scope (after the device is
+// idle), records their products in the statistics and frees the sets.
+int defer_flush() {
+  DeferScope &d = g_defer;
+  if (!d.n) return 0;
+  HIPCHK(hipDeviceSynchronize());
+  std::vector<unsigned int> w((size_t)d.n * MARGIN_SLOTS * MARGIN_STRIDE);
+  HIPCHK(hipMemcpy(w.data(), d.sets.p, w.size() * sizeof(unsigned int), hipMemcpyDeviceToHost));
+  for (int i = 0; i < d.n; i++) {
+    unsigned int b = 0;
+    for (int k = 0; k < MARGIN_SLOTS; k++) b = std::max(b, w[((size_t)i * MARGIN_SLOTS + k) * MARGIN_STRIDE]);
+    float f;
+    memcpy(&f, &b, sizeof f);
+    d.worst = std::max(d.worst, (double)f);
+    note_product(d.points[i], d.bits[i], (double)f);
+  }
+  d.n = 0;
+  return 0;
+}
+
+int mul_dev_body(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL, int force_bits,
+                 hipStream_t s, bcg_mul_info *info, const uint32_t *D, size_t DL);
+
 int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL,
                    int force_bits, hipStream_t s, bcg_mul_info *info, const uint32_t *D = nullptr, size_t DL = 0) {
+  DeferScope &d = g_defer;
+  if (!d.open || d.in_product || g_fixed || std::this_thread::get_id() != d.owner || C == A || C == B)
+    return mul_dev_body(C, CL, A, AL, B, BL, force_bits, s, info, D, DL);
+  // a deferred product: margin slots of its own set, no read-back
+  if (d.n == DEFER_SETS)
+    if (int rc = defer_flush()) return rc;
+  void *keep = g.margin.p;
+  g.margin.p = (unsigned char *)d.sets.p + (size_t)d.n * MARGIN_SET_BYTES;
+  d.in_product = true;
+  bcg_mul_info inf;
+  memset(&inf, 0, sizeof inf);
+  const int rc = mul_dev_body(C, CL, A, AL, B, BL, force_bits, s, &inf, D, DL);
+  d.in_product = false;
+  g.margin.p = keep;
+  if (rc) return rc;
+  if (inf.points) {  // an FFT product: its set holds its margin (schoolbook products have none)
+    d.points[d.n] = inf.points;
+    d.bits[d.n] = inf.bits;
+    d.n++;
+  }
+  if (info) *info = inf;
+  return 0;
+}
+
+int mul_dev_body(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const uint32_t *B, size_t BL, int force_bits,
+                 hipStream_t s, bcg_mul_info *info, const uint32_t *D, size_t DL) {
   bcg_mul_info inf;
   memset(&inf, 0, sizeof inf);
   bool square = (A == B && AL == BL);
@@ -1290,9 +1364,30 @@ int mul_dev_locked(uint32_t *C, size_t CL, const uint32_t *A, size_t AL, const u
     }
     break;
   }
-  note_product(inf.points, inf.bits, inf.margin);
+  if (!g_defer.in_product) note_product(inf.points, inf.bits, inf.margin);  // deferred: when its set is read
   if (info) *info = inf;
   return 0;
+}
+
+int defer_begin() {
+  DeferScope &d = g_defer;
+  if (d.open) return fail(BCG_E_ARG, "a deferral scope is already open");
+  if (int rc = ensure(d.sets, DEFER_SETS * MARGIN_SET_BYTES)) return rc;
+  d.open = true;
+  d.owner = std::this_thread::get_id();
+  d.n = 0;
+  d.worst = 0.0;
+  return 0;
+}
+
+int defer_end(double *worst) {
+  DeferScope &d = g_defer;
+  if (!d.open || std::this_thread::get_id() != d.owner) return fail(BCG_E_ARG, "no deferral scope open on this thread");
+  const int rc = defer_flush();
+  d.open = false;
+  d.n = 0;
+  if (worst) *worst = d.worst;
+  return rc;
 }
 
 // ------------------------------------------------- reference-order transforms
@@ -1631,7 +1726,10 @@ int bcg_shutdown(void) {
   std::lock_guard<std::recursive_mutex> lk(g_lock);
   if (!g.ready) return 0;
   (void)hipStreamSynchronize(g.stream);
-  for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf, &g.desc, &g.counters}) {
+  g_defer.open = false;
+  g_defer.n = 0;
+  for (DevBuf *b : {&g.bufA, &g.bufB, &g.wA, &g.wB, &g.wC, &g.blk, &g.margin, &g.zbuf, &g.desc, &g.counters,
+                    &g_defer.sets}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->bytes = 0;
@@ -2737,6 +2835,16 @@ int bcg_stats_reset(void) {
   std::lock_guard<std::recursive_mutex> lk(g_lock);
   g_stats = bcg_stats();
   return 0;
+}
+
+int bcg_defer_begin(void) {
+  ENTER();
+  return defer_begin();
+}
+
+int bcg_defer_end(double *worst_margin) {
+  ENTER();
+  return defer_end(worst_margin);
 }
 
 int bcg_stats_get(bcg_stats *out) {

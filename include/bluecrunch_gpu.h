@@ -281,6 +281,24 @@ typedef struct {
 int bcg_stats_reset(void);
 int bcg_stats_get(bcg_stats *out);
 
+/* Deferred margin checks, for device-resident callers that queue products
+ * back to back. No counterpart in the reference (its fft_to_int checks
+ * nothing). Between bcg_defer_begin() and bcg_defer_end(), the calling
+ * thread's products (bcg_mul_dev, bcg_muladd_dev and the single products
+ * of the e pipeline and the device bigfloats) neither read their rounding
+ * margin back nor re-run at
+ * >= 0.25: no host synchronisation per product, bcg_mul_info.margin reads 0.
+ * Each keeps its margin on the device (sets of 64 products, read together
+ * when they run out); bcg_defer_end waits for the device and returns the
+ * worst margin of the scope. Results of the scope are exact only if that
+ * is < 0.25; otherwise the caller redoes its work outside a scope, where
+ * every product checks and re-runs itself. Products whose output aliases
+ * an input, batched calls and other threads' products check at once as
+ * usual. One scope at a time (BCG_E_ARG otherwise); bcg_defer_end must come
+ * from the thread that opened it. */
+int bcg_defer_begin(void);
+int bcg_defer_end(double *worst_margin);
+
 #ifdef __cplusplus
 }
 #endif

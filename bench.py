@@ -59,7 +59,17 @@ def algorithmic_bytes(kind, info, al, bl, cl):
         return 16 * n + carry_in  # read the product array, write coefficients / group sums
     if kind == "carry":
         return carry_in + 4 * cl  # read coefficients / group sums, write words
+    if kind == "middle":
+        # chunked middle section: the second forward, middle and second inverse
+        # passes of every group (one span per product)
+        return sum(algorithmic_bytes(k, info, al, bl, cl) for k in ("fwd", "mid", "inv"))
     return 0
+
+
+# the library's default chunking of three-pass middle passes (bcg_plan_chunk):
+# 128 MiB groups on two streams
+CHUNK_BYTES = 128 << 20
+CHUNK_STREAMS = 2
 
 
 def read_traffic(kind, words):
@@ -446,15 +456,16 @@ def main():
             step_sq()
             ok = ok and bool(product_fingerprint_ok(a, a, dS.to_host()))
 
-    kinds = ("fwd_pack", "fwd", "mid", "inv", "inv_unpack", "carry")
+    kinds = ("fwd_pack", "fwd", "mid", "inv", "inv_unpack", "carry", "middle")
 
-    def read_kernels(steps):
+    def read_kernels(steps, inf=None):
+        inf = inf or info
         out = {}
         for kind in kinds:
             n, ms = bc.prof_read(kind)
             if n:
                 per = ms / n
-                byt = algorithmic_bytes(kind, info, al, bl, cl) * steps / n  # per launch
+                byt = algorithmic_bytes(kind, inf, al, bl, cl) * steps / n  # per launch
                 out[kind] = {"launches": n, "avg_ms": per, "alg_bytes": byt, "GBps": byt / (per * 1e-3) / 1e9}
         return out
 
@@ -469,7 +480,27 @@ def main():
     bc.prof_enable(False)
     kern = read_kernels(prof_steps)
     bc.prof_reset()
-    dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches"]) if kern else None
+    # with chunked middle passes the launches of the two streams overlap and a
+    # launch's span is no measure of its kernel: the same kernels are also
+    # timed over whole transforms (chunking off), one launch per pass
+    kern_whole = None
+    if info.get("chunks"):
+        bc.plan_chunk(0)
+        bc.prof_enable(True)
+        for _ in range(prof_steps):
+            info_whole = step()
+        bc.sync()
+        bc.prof_enable(False)
+        kern_whole = read_kernels(prof_steps, info_whole)
+        bc.prof_reset()
+        bc.plan_chunk(CHUNK_BYTES, CHUNK_STREAMS)
+    # the dominant unit: the chunked middle section when the product runs its
+    # middle passes in groups on two streams (their launches overlap, so a
+    # launch's own span is no measure of it), else the costliest kernel kind
+    if "middle" in kern:
+        dom = "middle"
+    else:
+        dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches"]) if kern else None
 
     # timed region: K products; only the dominant kernel kind carries HIP events
     # (recorded on its launch stream), so the events cost the wall time little
@@ -510,9 +541,12 @@ def main():
         ach = d["alg_bytes"] / (d["avg_ms"] * 1e-3)
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": ach / HBM_PEAK, "traffic": read_traffic(dom, al)}
+        if dom == "middle":
+            roof["span"] = (f"second forward, middle and second inverse pass kernels of {info['chunks']} groups "
+                            "on two streams, fork to join, one span per product")
     # FFT passes (everything but the carry) per product: their bytes over the
     # product's wall time less the carry kernels (passes may overlap in time)
-    fft_bytes = sum(v["alg_bytes"] * v["launches"] for k, v in kern.items() if k != "carry") / prof_steps
+    fft_bytes = sum(v["alg_bytes"] * v["launches"] for k, v in kern.items() if k not in ("carry", "middle")) / prof_steps
     carry_ms = kern["carry"]["avg_ms"] * kern["carry"]["launches"] / prof_steps if "carry" in kern else 0.0
 
     # host arrays in and out (PCIe copies included): never the headline value
@@ -575,6 +609,9 @@ def main():
             "note": "bcg_mul_host: operands copied in and the product copied out over PCIe (8 MiB chunks through pinned staging; the product into a fresh array each call)"},
         "fft_passes_hbm_frac": (fft_bytes / (sec - carry_ms * 1e-3)) / HBM_PEAK if sec > carry_ms * 1e-3 else None,
         "kernels": kern,
+        # chunked products: the per-launch spans above overlap across the two
+        # streams; these are the same kernels over whole transforms
+        "kernels_whole_passes": kern_whole,
         "roofline": roof,
         "cpu_baseline": cpu,
         "e": e_leg,

@@ -31,7 +31,7 @@ EXPORTS = {
         "bcg_int_to_fft_dev", "bcg_fft_to_int_dev", "bcg_fft_forward_host", "bcg_fft_inverse_host",
         "bcg_fft_pointwise_host", "bcg_int_to_fft_host", "bcg_fft_to_int_host", "bcg_basecase_host",
         "bcg_tft_forward_host", "bcg_tft_inverse_host", "bcg_plan_query", "bcg_plan_candidates",
-        "bcg_plan_force", "bcg_plan_current", "bcg_plan_digit_bias", "bcg_autotune_classes", "bcg_plan_force_batch", "bcg_plan_current_batch", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_test_fail_alloc", "bcg_bf_set", "bcg_bf_get", "bcg_bf_free", "bcg_bf_add", "bcg_bf_sub", "bcg_bf_mul", "bcg_bf_rcp", "bcg_bf_div", "bcg_prof_enable", "bcg_prof_select", "bcg_prof_read",
+        "bcg_plan_force", "bcg_plan_current", "bcg_plan_digit_bias", "bcg_plan_chunk", "bcg_autotune_classes", "bcg_plan_force_batch", "bcg_plan_current_batch", "bcg_autotune", "bcg_e_terms", "bcg_e_bsr_dev", "bcg_div_fraction_dev", "bcg_radix_dev", "bcg_e_digits", "bcg_e_digits_ex", "bcg_test_fail_alloc", "bcg_bf_set", "bcg_bf_get", "bcg_bf_free", "bcg_bf_add", "bcg_bf_sub", "bcg_bf_mul", "bcg_bf_rcp", "bcg_bf_div", "bcg_prof_enable", "bcg_prof_select", "bcg_prof_read",
         "bcg_prof_reset", "bcg_stats_reset", "bcg_stats_get", "bcg_defer_begin", "bcg_defer_end",
     ],
     "fft.h": [
@@ -48,14 +48,14 @@ EXPORTS = {
 }
 
 PROF_KINDS = {
-    "fwd_pack": 0, "fwd": 1, "mid": 2, "inv": 3, "inv_unpack": 4, "carry": 5, "other": 6,
+    "fwd_pack": 0, "fwd": 1, "mid": 2, "inv": 3, "inv_unpack": 4, "carry": 5, "other": 6, "middle": 7,
 }
 
 
 class MulInfo(ctypes.Structure):
     _fields_ = [("bits", ctypes.c_int), ("passes", ctypes.c_int), ("points", _sz), ("margin", ctypes.c_double),
                 ("retries", ctypes.c_int), ("square", ctypes.c_int), ("group_digits", ctypes.c_int),
-                ("group_stride", ctypes.c_int)]
+                ("group_stride", ctypes.c_int), ("chunks", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -124,6 +124,7 @@ def lib():
     L.bcg_plan_candidates.argtypes = [_sz, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
     L.bcg_plan_force.argtypes = [_sz, i]
     L.bcg_plan_digit_bias.argtypes = [i]
+    L.bcg_plan_chunk.argtypes = [_sz, i]
     L.bcg_autotune_classes.argtypes = [_sz, _sz, i, ctypes.POINTER(i)]
     L.bcg_plan_force_batch.argtypes = [_sz, i]
     L.bcg_plan_current_batch.argtypes = [_sz]
@@ -493,6 +494,12 @@ def plan_digit_bias(extra_bits):
     """Plan digit sizes up to extra_bits above the rounding bound (0: normal);
     exercises the rounding-margin retry."""
     check(lib().bcg_plan_digit_bias(extra_bits), "plan_digit_bias")
+
+
+def plan_chunk(nbytes, streams=2):
+    """Chunked middle passes of three-pass products: group size in bytes of
+    operand points (0: whole-transform passes) and 1 or 2 streams."""
+    check(lib().bcg_plan_chunk(int(nbytes), int(streams)), "plan_chunk")
 
 
 def plan_current(n):

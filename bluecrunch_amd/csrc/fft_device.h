@@ -52,6 +52,20 @@ __device__ __forceinline__ void st_nt(cd *p, cd v) {
   w.y = v.y;
   __builtin_nontemporal_store(w, reinterpret_cast<nt_d2 *>(p));
 }
+// Cache policy of a pass: ordinary loads and stores where the next launch
+// reads the points back while they are still in the memory-side cache (the
+// chunked middle passes of a three-pass product), streaming ones otherwise.
+// CACHE bit 0: ordinary loads; bit 1: ordinary stores.
+template <bool PLAIN>
+__device__ __forceinline__ cd ld_pol(const cd *p) {
+  if constexpr (PLAIN) return *p;
+  else return ld_nt(p);
+}
+template <bool PLAIN>
+__device__ __forceinline__ void st_pol(cd *p, cd v) {
+  if constexpr (PLAIN) *p = v;
+  else st_nt(p, v);
+}
 // e^{+2 pi i j / 16}, j = 0..15, to 20 significant digits.
 #define BCG_C1 0.92387953251128675613
 #define BCG_R2 0.70710678118654752440

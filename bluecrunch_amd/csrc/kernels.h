@@ -385,23 +385,23 @@ struct PackDigits {
 // base address, plus the row step S t M, which is the same across the wave
 // (scalar registers), so each access costs one address addition and the
 // sixteen accesses hold no per-row address registers.
-template <int K, int S>
+template <int K, int S, bool PLAIN = false>
 __device__ __forceinline__ void load_rows16(const cd *base, cd (&v)[16], int r0, int c, unsigned long long m0,
                                             unsigned long long M) {
   const cd *lb = base + (unsigned long long)(K * r0) * M + m0 + c;
 #pragma unroll
-  for (int t = 0; t < 16; t++) v[t] = ld_nt(lb + (unsigned long long)(S * t) * M);
+  for (int t = 0; t < 16; t++) v[t] = ld_pol<PLAIN>(lb + (unsigned long long)(S * t) * M);
 }
 
-template <int K, int S>
+template <int K, int S, bool PLAIN = false>
 __device__ __forceinline__ void store_rows16(cd *base, const cd (&v)[16], int r0, int c, unsigned long long m0,
                                              unsigned long long M) {
   cd *lb = base + (unsigned long long)(K * r0) * M + m0 + c;
 #pragma unroll
-  for (int t = 0; t < 16; t++) st_nt(lb + (unsigned long long)(S * t) * M, v[t]);
+  for (int t = 0; t < 16; t++) st_pol<PLAIN>(lb + (unsigned long long)(S * t) * M, v[t]);
 }
 
-template <int L, int C, bool PACK>
+template <int L, int C, bool PACK, int CACHE = 0>
 __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   using TC = Tile<L, C>;
   using ST = SplitTile<L, C>;
@@ -476,7 +476,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
         const int h = i + t * PER0;
         x[t] = cmul(make_double2((double)dre[h], (double)dim_[h]), cmul(wpack, w64(h)));
       } else {
-        x[t] = ld_nt(&base[row * a.M + m]);
+        x[t] = ld_pol<(CACHE & 1) != 0>(&base[row * a.M + m]);
       }
     }
     dif_levels<RB0>(x, a.tl, q, 1);
@@ -505,7 +505,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
       const int row = r0 + h * (L / 16);
       cd v = lds[slot<C>(row, c)];
       if (a.M > 1) v = cmul(v, w.at(h));
-      st_nt(&base[(unsigned long long)row * a.M + m], v);
+      st_pol<(CACHE & 2) != 0>(&base[(unsigned long long)row * a.M + m], v);
     }
   }
 }
@@ -519,7 +519,7 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
 // is not held to the registers of the coefficient stores.
 // The unpacking pass of a split tile is held to 168 VGPRs (three waves per
 // SIMD; measured faster than two waves without the two spilled registers).
-template <int L, int C, int MODE>
+template <int L, int C, int MODE, int CACHE = 0>
 __global__ __launch_bounds__((Tile<L, C>::T))
 __attribute__((amdgpu_waves_per_eu((MODE == 1 && SplitTile<L, C>::ON && C >= 8) ? 3 : 1, 8))) void k_inv(PassArgs a) {
   constexpr bool UNPACK = MODE != 0;
@@ -543,7 +543,7 @@ __attribute__((amdgpu_waves_per_eu((MODE == 1 && SplitTile<L, C>::ON && C >= 8) 
     // rows 16 r0 + t (frequency 16 brev4(t) + brev4(r0)) with the conjugate
     // pass twiddle, stage 1 in registers, transpose to rows r0 + 16 t, stage 0
     cd x[16], y[16], tb0[4];
-    load_rows16<16, 1>(base, y, r0, c, m0, a.M);
+    load_rows16<16, 1, (CACHE & 1) != 0>(base, y, r0, c, m0, a.M);
     Walk16 w;
     if (a.M > 1) {
       w.template init_shared<C>(a.roots, a.tw_mul * m * brev((unsigned)r0, 4), a.tw_mul * m * 16ull);
@@ -557,7 +557,7 @@ __attribute__((amdgpu_waves_per_eu((MODE == 1 && SplitTile<L, C>::ON && C >= 8) 
     split_transpose<C>(ld, y, x, r0, c, true);
     dit_levels<4>(x, tb0);
     if constexpr (!UNPACK) {
-      store_rows16<1, 16>(base, x, r0, c, m0, a.M);
+      store_rows16<1, 16, (CACHE & 2) != 0>(base, x, r0, c, m0, a.M);
     } else {
       double worst = 0.0;
       const cd wun = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
@@ -603,7 +603,7 @@ __attribute__((amdgpu_waves_per_eu((MODE == 1 && SplitTile<L, C>::ON && C >= 8) 
     // all sixteen loads in flight first, then the conjugate pass twiddle
     // (rows r0 + h * L/16 as in k_fwd) on the way into LDS
     cd xs[16];
-    load_rows16<1, L / 16>(base, xs, r0, c, m0, a.M);
+    load_rows16<1, L / 16, (CACHE & 1) != 0>(base, xs, r0, c, m0, a.M);
     Walk16 w;
     if (a.M > 1) w.template init_shared<C>(a.roots, a.tw_mul * m * 16ull * brev((unsigned)r0, TC::LOG - 4), a.tw_mul * m);
 #pragma unroll
@@ -645,7 +645,7 @@ __attribute__((amdgpu_waves_per_eu((MODE == 1 && SplitTile<L, C>::ON && C >= 8) 
           worst = fmax(worst, round_store(a, f, row, m, L));
         }
       } else {
-        st_nt(&base[row * a.M + m], x[t]);
+        st_pol<(CACHE & 2) != 0>(&base[row * a.M + m], x[t]);
       }
     }
   }
@@ -662,7 +662,7 @@ __attribute__((amdgpu_waves_per_eu((MODE == 1 && SplitTile<L, C>::ON && C >= 8) 
 
 // Forward DIF of one contiguous block of L points, ending with the innermost
 // stage's outputs in registers: xr[i * R + t] for butterfly i of that stage.
-template <int L, bool PACK>
+template <int L, bool PACK, int CACHE = 0>
 __device__ __forceinline__ void mid_forward(const PassArgs &a, cd *lds, const cd *__restrict__ src,
                                             const uint32_t *__restrict__ w, unsigned long long wl,
                                             unsigned long long blk0, cd (&xr)[Tile<L, 1>::EPT], int tid) {
@@ -679,7 +679,7 @@ __device__ __forceinline__ void mid_forward(const PassArgs &a, cd *lds, const cd
 #pragma unroll
     for (int t = 0; t < R0; t++) {
       unsigned long long j = blk0 + (unsigned long long)(q + t * SUB0);
-      x[t] = PACK ? packed_point(a, w, wl, j) : ld_nt(&src[j]);
+      x[t] = PACK ? packed_point(a, w, wl, j) : ld_pol<(CACHE & 1) != 0>(&src[j]);
     }
     dif_levels<RB0>(x, a.tl, q, 1);
     if (TC::NST2 == 1 && !TC::THREE) {
@@ -734,15 +734,15 @@ __device__ __forceinline__ void mid_forward(const PassArgs &a, cd *lds, const cd
 // pass on one contiguous block of L points. Ends with the outermost inverse
 // stage's outputs in registers: out[i * R0 + t] holds row q + t * (L / R0),
 // q = tid + i * T (natural order). The caller stores or unpacks them.
-template <int L, bool SQUARE, bool PACK>
+template <int L, bool SQUARE, bool PACK, int CACHE = 0>
 __device__ __forceinline__ void mid_pipeline(const PassArgs &a, cd *lds, unsigned long long blk0,
                                              cd (&xa)[Tile<L, 1>::EPT], int tid) {
   using TC = Tile<L, 1>;
-  mid_forward<L, PACK>(a, lds, a.data, a.wa, a.al, blk0, xa, tid);
+  mid_forward<L, PACK, CACHE>(a, lds, a.data, a.wa, a.al, blk0, xa, tid);
   if (!SQUARE) {
     cd xb[TC::EPT];
     __syncthreads();
-    mid_forward<L, PACK>(a, lds, a.data_b, a.wb, a.bl, blk0, xb, tid);
+    mid_forward<L, PACK, CACHE>(a, lds, a.data_b, a.wb, a.bl, blk0, xb, tid);
 #pragma unroll
     for (int i = 0; i < TC::EPT; i++) xa[i] = cmul(xa[i], xb[i]);
   } else {
@@ -815,7 +815,7 @@ __device__ __forceinline__ void mid_pipeline(const PassArgs &a, cd *lds, unsigne
 // workgroup (each with its own LDS tile): blocks of L < 1024 (1536 for 3 2^k)
 // have fewer than 64 threads, and one block per workgroup would leave most
 // lanes of the wave idle.
-template <int L, bool SQUARE, bool PACK, bool UNPACK, int BPW = 1>
+template <int L, bool SQUARE, bool PACK, bool UNPACK, int BPW = 1, int CACHE = 0>
 __global__ __launch_bounds__((Tile<L, 1>::T * BPW)) void k_mid(PassArgs a) {
   using TC = Tile<L, 1>;
   static_assert(BPW == 1 || !PACK, "one-pass products take one workgroup each");
@@ -826,7 +826,7 @@ __global__ __launch_bounds__((Tile<L, 1>::T * BPW)) void k_mid(PassArgs a) {
   cd *lds = lds_all + sub * TC::LDS_ELEMS;
   const unsigned long long blk0 = ((unsigned long long)blockIdx.x * BPW + sub) * L;
   cd xa[TC::EPT];
-  mid_pipeline<L, SQUARE, PACK>(a, lds, blk0, xa, tid);
+  mid_pipeline<L, SQUARE, PACK, CACHE>(a, lds, blk0, xa, tid);
   constexpr int R0 = 1 << TC::FIRSTB;
   constexpr int PER0 = TC::EPT / R0;
   constexpr int SUB0 = L / R0;
@@ -840,7 +840,7 @@ __global__ __launch_bounds__((Tile<L, 1>::T * BPW)) void k_mid(PassArgs a) {
       if (UNPACK)
         worst = fmax(worst, unpack_point(a, xa[i * R0 + t], j));
       else
-        st_nt(&a.data[j], xa[i * R0 + t]);
+        st_pol<(CACHE & 2) != 0>(&a.data[j], xa[i * R0 + t]);
     }
   }
   if (UNPACK) margin_reduce(a.margin, worst, a.margin_mask);

@@ -20,6 +20,13 @@ fwd_fn pick_inv_pow2(int L, int C);
 fwd_fn pick_inv_three(int L, int C);
 mid_fn pick_mid(int L);
 mul1_fn pick_mul1(int L);
+// Cache-keeping variants for the chunked middle passes of three-pass
+// products (forward pass with ordinary stores, middle pass with ordinary
+// loads and stores, inverse pass with ordinary loads); nullptr when that
+// tile has none.
+fwd_fn pick_fwd_keep(int L, int C);
+fwd_fn pick_inv_keep(int L, int C);
+mid_fn pick_mid_keep(int L);
 
 // Code-object preloads, one per launch_*.hip unit (called at initialisation).
 int preload_mul1();
@@ -28,6 +35,7 @@ int preload_fwd_three();
 int preload_inv_pow2();
 int preload_inv_three();
 int preload_mid();
+int preload_keep();
 
 inline fwd_fn pick_fwd(int L, int C) { return (L % 3 == 0) ? pick_fwd_three(L, C) : pick_fwd_pow2(L, C); }
 inline fwd_fn pick_inv(int L, int C) { return (L % 3 == 0) ? pick_inv_three(L, C) : pick_inv_pow2(L, C); }

@@ -36,6 +36,7 @@ typedef struct {
   int square;        /* 1 when the squaring path ran (one forward transform) */
   int group_digits;  /* carry input: digits per exact group sum (8 or 16), 0 = 64-bit coefficients */
   int group_stride;  /* words stored per group sum (0 with coefficients) */
+  int chunks;        /* groups the middle passes ran in (0: whole-transform passes) */
 } bcg_mul_info;
 
 /* Device set-up: select device, create the stream, warm the kernels.
@@ -160,6 +161,14 @@ int bcg_plan_force(size_t N, int index);
  * bias the first attempt of a product tends to reach a margin >= 0.25 and is
  * re-run with smaller digits, which bcg_stats counts as retries. */
 int bcg_plan_digit_bias(int extra_bits);
+/* Chunked middle passes of three-pass products: the second forward pass, the
+ * middle pass and the second inverse pass run group by group of blocks, each
+ * group (`bytes` of operand points) kept in the GPU's memory-side cache from
+ * one of its launches to the next; 0 runs each pass over the whole transform.
+ * streams 2 alternates groups between the call's stream and a side stream
+ * (joined back before the product continues), 1 keeps them on one stream.
+ * Defaults: 128 MiB, 2. No counterpart in the reference (a tuning knob). */
+int bcg_plan_chunk(size_t bytes, int streams);
 /* Index of the candidate transforms of N points use now (forced, autotuned
  * or default); negative BCG_E* code when N has no split. */
 int bcg_plan_current(size_t N);
@@ -258,6 +267,9 @@ int bcg_test_fail_alloc(long nth);
 #define BCG_K_INV_UNPACK 4 /* last inverse pass, rounding to coefficients */
 #define BCG_K_CARRY 5     /* coefficient -> word carry resolve */
 #define BCG_K_OTHER 6
+#define BCG_K_MIDDLE 7    /* chunked middle section of a 3-pass product: every group's second
+                            forward, middle and second inverse pass, timed as one span from
+                            the fork of the side streams to their join */
 int bcg_prof_enable(int on);
 /* Bit mask of the kinds bracketed while enabled (bit k = kind k; default all):
  * timing one kind keeps the event overhead off the others. */

@@ -233,6 +233,70 @@ def test_every_pass_split_is_exact(bc, words, stride):
         bc.plan_force(n, -1)
 
 
+@pytest.mark.parametrize("words,group_bytes,exact", [(500000, [1, 5 << 20], True), (1 << 21, [24 << 20], False)])
+def test_chunked_middle_passes(bc, words, group_bytes, exact):
+    # three-pass splits run their middle passes group by group of blocks (the
+    # points of a group stay in the memory-side cache between launches). With
+    # small groups (one block each, or several with a short last group) every
+    # three-pass split must give the words of whole-transform passes, for
+    # products and squares, and those words the exact product (checked by
+    # Python integers at 500000 words, by fingerprint at 2^21 words)
+    a, b = operands(77 + words, words, words)
+    n = bc.plan(words, words)["points"]
+    cands = bc.plan_candidates(n)
+    picks = [i for i, (P, L, C) in enumerate(cands) if P == 3]
+    assert picks
+    want = (exact_product(a, b), exact_product(a, a)) if exact else None
+    chunked = 0
+    try:
+        for i in picks:
+            bc.plan_force(n, i)
+            bc.plan_chunk(0)
+            c0, info0 = bc.mul(a, b)
+            s0, _ = bc.mul(a, a)
+            assert info0["chunks"] == 0
+            if exact:
+                assert np.array_equal(c0, want[0]), (i, cands[i])
+                assert np.array_equal(s0, want[1]), (i, cands[i])
+            else:
+                assert product_fingerprint_ok(a, b, c0) and product_fingerprint_ok(a, a, s0), (i, cands[i])
+            for gb, streams in [(g, k) for g in group_bytes for k in (1, 2)]:
+                bc.plan_chunk(gb, streams)
+                c, info = bc.mul(a, b)
+                s, sinfo = bc.mul(a, a)
+                assert np.array_equal(c, c0), (i, cands[i], gb, streams, info)
+                assert np.array_equal(s, s0), (i, cands[i], gb, streams, sinfo)
+                assert info["margin"] < 0.25 and sinfo["margin"] < 0.25
+                chunked += info["chunks"] > 1 and sinfo["chunks"] > 1
+    finally:
+        bc.plan_chunk(128 << 20)
+        bc.plan_force(n, -1)
+    assert chunked
+
+
+def test_chunked_groups_at_2p24_words(bc):
+    # the 2^24-word product with the 256 x 256 x 1024 split (2^26 points)
+    # runs its middle passes in groups at the default group size; its words
+    # equal those of whole-transform passes and pass the fingerprint check
+    a, b = operands(1, 1 << 24, 1 << 24)
+    n = bc.plan(1 << 24, 1 << 24)["points"]
+    cands = bc.plan_candidates(n)
+    idx = [i for i, (P, L, C) in enumerate(cands) if L == [256, 256, 1024] and C == [16, 16, 1]]
+    assert idx, cands
+    try:
+        bc.plan_force(n, idx[0])
+        c, info = bc.mul(a, b)
+        assert info["passes"] == 3 and info["chunks"] > 1, info
+        assert product_fingerprint_ok(a, b, c)
+        bc.plan_chunk(0)
+        c0, info0 = bc.mul(a, b)
+    finally:
+        bc.plan_chunk(128 << 20)
+        bc.plan_force(n, -1)
+    assert info0["chunks"] == 0
+    assert np.array_equal(c, c0)
+
+
 def test_autotune_keeps_exactness(bc):
     chosen, times = bc.autotune(300000, 300000, reps=1)
     assert 0 <= chosen < len(times) and all(t > 0 for t in times)

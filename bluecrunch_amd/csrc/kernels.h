@@ -1416,9 +1416,11 @@ __device__ __forceinline__ unsigned long long word_partial_lds(const long long *
 }
 
 // Words out[0..CL) = (sum_j coef[j] 2^{bits j}) mod 2^{32 CL}, coefficients in
-// LDS, carries resolved with a workgroup scan of carry functions.
+// LDS, carries resolved with a workgroup scan of carry functions; 64-bit word
+// indices, every word's partial computed twice (for output lengths of 2^19
+// words and more: a short product with a long addend).
 template <int T>
-__device__ void wg_carry(const long long *coef, long long ncoef, int bits, int cbits, uint32_t *out, long long CL,
+__device__ void wg_carry_wide(const long long *coef, long long ncoef, int bits, int cbits, uint32_t *out, long long CL,
                          const uint32_t *add, long long addl, int tid) {
   __shared__ unsigned wave_fn[T / 64 + 1];
   const long long per = (CL + T - 1) / T;
@@ -1465,6 +1467,93 @@ __device__ void wg_carry(const long long *coef, long long ncoef, int bits, int c
     out[w] = (uint32_t)v;
     cin = (unsigned)(v >> 32);
     prev = P;
+  }
+}
+
+// x / b for 0 <= x < 2^24 and 1 <= b <= 64 through the float reciprocal,
+// corrected to the exact floor.
+__device__ __forceinline__ int fdiv_small(int x, int b, float inv_b) {
+  int q = (int)((float)x * inv_b);
+  if (q * b > x) q--;
+  else if ((q + 1) * b <= x) q++;
+  return q;
+}
+
+// P_w for word w < 2^19 from the coefficients in LDS, 32-bit index arithmetic:
+// coefficient j (below 2^cbits) sits at bit b j, so words take j in
+// [ceil((32 w - cbits + 1) / b), floor((32 w + 31) / b)].
+__device__ __forceinline__ unsigned long long word_partial32(const long long *coef, int ncoef, int bits, int cbits,
+                                                             float inv_b, int w) {
+  if (w < 0) return 0ull;
+  const int lo_bit = 32 * w - cbits + 1;
+  const int jlo = lo_bit <= 0 ? 0 : fdiv_small(lo_bit + bits - 1, bits, inv_b);
+  int jhi = fdiv_small(32 * w + 31, bits, inv_b);
+  if (jhi >= ncoef) jhi = ncoef - 1;
+  unsigned long long P = 0;
+  int sh = bits * jlo - 32 * w;  // in (-cbits, 32)
+  for (int j = jlo; j <= jhi; j++, sh += bits) {
+    const unsigned long long c = (unsigned long long)coef[j];
+    P += (unsigned long long)(uint32_t)(sh >= 0 ? c << sh : c >> -sh);
+  }
+  return P;
+}
+
+// Words out[0..CL) = (sum_j coef[j] 2^{bits j} + add) mod 2^{32 CL}, coefficients
+// in LDS. Each thread takes CL / T consecutive words and writes them as for a
+// carry-in of 0, computing every word's partial once, while it tracks its
+// carry outs for carry-ins 0 and 1; a workgroup scan of those carry functions
+// gives each thread its carry-in, and a carry-in of 1 then ripples through the
+// thread's leading all-ones words.
+template <int T>
+__device__ void wg_carry(const long long *coef, long long ncoef, int bits, int cbits, uint32_t *out, long long CL,
+                         const uint32_t *add, long long addl, int tid) {
+  if (CL >= (1ll << 19) || ncoef >= (1ll << 30)) {  // 32 w + 31 < 2^24 below
+    wg_carry_wide<T>(coef, ncoef, bits, cbits, out, CL, add, addl, tid);
+    return;
+  }
+  __shared__ unsigned wave_fn[T / 64 + 1];
+  const int cl = (int)CL, nc = (int)ncoef;
+  const float inv_b = 1.0f / (float)bits;
+  const int per = (cl + T - 1) / T;
+  const int w0 = tid * per;
+  const int w1 = w0 + per < cl ? w0 + per : cl;
+  unsigned f = 2u;  // identity
+  {
+    unsigned long long prev = word_partial32(coef, nc, bits, cbits, inv_b, w0 - 1);
+    if (add && w0 >= 1 && w0 - 1 < addl) prev += add[w0 - 1];
+    unsigned c0 = 0, c1 = 1;
+    for (int w = w0; w < w1; w++) {
+      unsigned long long P = word_partial32(coef, nc, bits, cbits, inv_b, w);
+      if (add && w < addl) P += add[w];
+      const unsigned long long v = (P & 0xffffffffull) + (prev >> 32);
+      const unsigned long long s0 = v + c0;
+      out[w] = (uint32_t)s0;
+      c0 = (unsigned)(s0 >> 32);
+      c1 = (unsigned)((v + c1) >> 32);
+      prev = P;
+    }
+    if (w0 < w1) f = c0 | (c1 << 1);
+  }
+  // inclusive scan within the wave
+  const int lane = tid & 63, wave = tid >> 6;
+  unsigned inc = f;
+  for (int off = 1; off < 64; off <<= 1) {
+    unsigned g = __shfl_up(inc, off, 64);
+    if (lane >= off) inc = carry_compose(g, inc);
+  }
+  unsigned excl = __shfl_up(inc, 1, 64);
+  if (lane == 0) excl = 2u;
+  if (lane == 63) wave_fn[wave] = inc;
+  __syncthreads();
+  unsigned pre = 2u;
+  for (int w = 0; w < wave; w++) pre = carry_compose(pre, wave_fn[w]);
+  const unsigned cin = carry_compose(pre, excl) & 1u;  // carry into word w0 (block carry-in is 0)
+  if (cin) {
+    for (int w = w0; w < w1; w++) {
+      const uint32_t x = out[w] + 1u;
+      out[w] = x;
+      if (x) break;
+    }
   }
 }
 

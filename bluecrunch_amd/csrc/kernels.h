@@ -962,23 +962,30 @@ __device__ __forceinline__ long long div_small(long long x, int b, double inv_b)
 
 constexpr int CARRY_GCOV = 4;  // most group sums covering one word
 
-// P_w of one word from the LDS window cl[] (coefficient jbase at cl[0]).
-__device__ __forceinline__ unsigned long long partial_from_lds(const long long *cl, long long jbase, long long ncoef,
-                                                               int bits, int cbits, double inv_b, long long w) {
-  if (w < 0) return 0ull;
-  long long lo_bit = 32 * w - cbits + 1;
-  long long jlo = lo_bit <= 0 ? 0 : div_small(lo_bit + bits - 1, bits, inv_b);
-  long long jhi = div_small(32 * w + 31, bits, inv_b);
-  if (jhi >= ncoef) jhi = ncoef - 1;
+// x / b for 0 <= x < 2^24 and 1 <= b <= 64 through the float reciprocal,
+// corrected to the exact floor.
+__device__ __forceinline__ int fdiv_small(int x, int b, float inv_b) {
+  int q = (int)((float)x * inv_b);
+  if (q * b > x) q--;
+  else if ((q + 1) * b <= x) q++;
+  return q;
+}
+
+// P_w of a block word from the staged window win[0..cnt) (coefficient jbase +
+// i at win[i]) in block-relative 32-bit arithmetic: x = 32 (w - W0) + D is the
+// word's lowest bit above bit b jbase (D = 32 W0 - b jbase, small), so every
+// numerator stays far below 2^24 and the float-reciprocal division is exact.
+__device__ __forceinline__ unsigned long long partial_rel(const long long *win, int cnt, int bits, int cbits,
+                                                          float inv_b, int x) {
+  const int lo = x - cbits + 1;
+  const int ilo = lo <= 0 ? 0 : fdiv_small(lo + bits - 1, bits, inv_b);
+  int ihi = x + 31 < 0 ? -1 : fdiv_small(x + 31, bits, inv_b);
+  if (ihi >= cnt) ihi = cnt - 1;
   unsigned long long P = 0;
-  int sh = (int)((long long)bits * jlo - 32 * w);
-  const long long *c = cl + (jlo - jbase);
-  for (long long j = jlo; j <= jhi; j++, sh += bits, c++) {
-    const unsigned long long v = (unsigned long long)*c;
-    if (sh >= 0)
-      P += (unsigned long long)(uint32_t)(v << sh);
-    else if (sh > -64)
-      P += (unsigned long long)(uint32_t)(v >> (-sh));
+  int sh = bits * ilo - x;  // in (-cbits, 32)
+  for (int i = ilo; i <= ihi; i++, sh += bits) {
+    const unsigned long long c = (unsigned long long)win[i];
+    P += (unsigned long long)(uint32_t)(sh >= 0 ? c << sh : c >> -sh);
   }
   return P;
 }
@@ -1022,17 +1029,19 @@ static __global__ __launch_bounds__(CT) void k_carry_local(CarryArgs a) {
   unsigned long long P[CWPT + 1];
   if (staged) {
     const int lane = tid & 63;
+    const int D = (int)(32 * W0 - (long long)a.bits * jbase);
+    const float inv_bf = 1.0f / (float)a.bits;
+    const int cnti = (int)cnt, x0 = 32 * tid * CWPT + D;  // lowest bit of word wfirst
 #pragma unroll
     for (int k = 1; k <= CWPT; k++) {
       long long w = wfirst - 1 + k;
-      P[k] = (w < (long long)a.CL) ? partial_from_lds(cl, jbase, (long long)a.ncoef, a.bits, a.cbits, inv_b, w) : 0ull;
+      P[k] = (w < (long long)a.CL) ? partial_rel(cl, cnti, a.bits, a.cbits, inv_bf, x0 + 32 * (k - 1)) : 0ull;
     }
     // word wfirst-1 is the left neighbour's last word
     unsigned long long left = __shfl_up(P[CWPT], 1, 64);
     if (lane == 0)
-      left = (wfirst - 1 < (long long)a.CL)
-                 ? partial_from_lds(cl, jbase, (long long)a.ncoef, a.bits, a.cbits, inv_b, wfirst - 1)
-                 : 0ull;
+      left = (wfirst >= 1 && wfirst - 1 < (long long)a.CL) ? partial_rel(cl, cnti, a.bits, a.cbits, inv_bf, x0 - 32)
+                                                          : 0ull;
     P[0] = left;
     if (a.add) {
 #pragma unroll
@@ -1468,15 +1477,6 @@ __device__ void wg_carry_wide(const long long *coef, long long ncoef, int bits, 
     cin = (unsigned)(v >> 32);
     prev = P;
   }
-}
-
-// x / b for 0 <= x < 2^24 and 1 <= b <= 64 through the float reciprocal,
-// corrected to the exact floor.
-__device__ __forceinline__ int fdiv_small(int x, int b, float inv_b) {
-  int q = (int)((float)x * inv_b);
-  if (q * b > x) q--;
-  else if ((q + 1) * b <= x) q++;
-  return q;
 }
 
 // P_w for word w < 2^19 from the coefficients in LDS, 32-bit index arithmetic:

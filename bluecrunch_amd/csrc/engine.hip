@@ -1262,16 +1262,23 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
   a.margin = dmargins;
   int fast = 1;  // every product of the class has coefficients below 2^51
   for (const MulDesc &d : hdesc) fast = fast && d.cbits <= 51;
-  for (int op = 0; op < 2; op++) {
+  // both operands' forward passes in one launch each (grid.y = 2 count) while
+  // grid.y allows it, else one launch per operand
+  const bool both = 2 * count <= 65535;
+  for (int op = 0; op < (both ? 1 : 2); op++) {
     for (int i = 0; i < p.P - 1; i++) {
       PassArgs b = a;
       if (int rc = pass_args(p, i, b)) return rc;
       b.data = op == 0 ? da : db;
       b.operand = op;
+      if (both) {
+        b.data_b = db;
+        b.pcount = count;
+      }
       fwd_fn f = pick_fwd(p.L[i], p.C[i]);
       if (!f) return fail(BCG_E_ARG, "no forward kernel for this tile");
       ProfScope ps(i == 0 ? BCG_K_FWD_PACK : BCG_K_FWD, s);
-      if (f(i == 0, b.ntiles, count, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
+      if (f(i == 0, b.ntiles, both ? 2 * count : count, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
       HIPCHK(hipGetLastError());
     }
   }

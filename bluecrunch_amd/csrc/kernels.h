@@ -42,6 +42,7 @@ struct PassArgs {
   uint32_t *ybuf;           // last pass, group mode: digits [C g, C g + C) summed into ybuf[g K .. g K + K)
   int ygK;                  // words per group sum (a multiple of 4: 16-byte stores)
   int pair;                 // forward passes of A and B in one launch: grid.y = 1 runs B (data_b, wb, bl)
+  int pcount;               // batched forward passes of both operands: grid.y = 2 pcount, rows >= pcount run B
 };
 
 // Batched launches (desc != nullptr): product p = blockIdx.y owns the arrays
@@ -55,7 +56,16 @@ __device__ __forceinline__ void bind_product(PassArgs &a) {
     }
     return;
   }
-  const unsigned long long p = blockIdx.y;
+  unsigned long long p = blockIdx.y;
+  if (a.pcount) {  // both operands in one launch: the upper half of grid.y runs B into data_b
+    const bool opb = p >= (unsigned long long)a.pcount;
+    if (opb) {
+      p -= (unsigned long long)a.pcount;
+      a.data = const_cast<cd *>(a.data_b);
+    }
+    a.operand = opb ? 1 : 0;
+    a.data_b = nullptr;
+  }
   const MulDesc &d = a.desc[p];
   a.wa = a.operand ? d.b : d.a;
   a.al = a.operand ? d.bl : d.al;

@@ -881,9 +881,10 @@ int read_margin(hipStream_t s, double *m) {
 // quarter of the bytes), and the carry reads those sums. Returns K, or 0 when
 // the plan keeps plain coefficients; group_stride rounds K up to 4 words.
 int group_stride(int K) { return (K + 3) & ~3; }
-int group_words(const Plan &p) {
-  if (p.P < 2 || (p.C[0] != 8 && p.C[0] != 16) || coef_bits(p) > 51) return 0;
-  const int G = p.C[0], b = p.bits, cb = coef_bits(p);
+// Words per exact group sum of G digits of b bits (coefficients below 2^cb),
+// 0 when the group-sum carry does not cover that shape.
+int group_words_for(int G, int b, int cb) {
+  if ((G != 8 && G != 16) || cb > 51) return 0;
   int off_max = 0;
   for (int g = 0; g < 32; g++) {
     int o = (G * b * g) & 31;
@@ -895,6 +896,8 @@ int group_words(const Plan &p) {
   if (cover > CARRY_GCOV || group_stride(K) > 8) return 0;
   return K;
 }
+
+int group_words(const Plan &p) { return p.P < 2 ? 0 : group_words_for(p.C[0], p.bits, coef_bits(p)); }
 
 // Coefficients of plan p are written transposed when the last pass has
 // narrow column tiles (see unpack_point); the carry reads them back that way.
@@ -1294,12 +1297,33 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
     if (f(false, false, (int)(p.N / (unsigned long long)p.L[i]), count, s, b)) return fail(BCG_E_ARG, "kernel rejected this pass");
     HIPCHK(hipGetLastError());
   }
+  // group mode when every product of the class has exact group sums: the last
+  // inverse pass writes them (one stride for the class, the widest product's)
+  // into the B buffer and the group-sum carry resolves them per product
+  bool groups = fast && p.P >= 2 && (p.C[0] == 8 || p.C[0] == 16);
+  int gks = 0, gcov = 0, bmin = 64;
+  for (const MulDesc &d : hdesc) {
+    if (!d.gk) groups = false;
+    gks = std::max(gks, group_stride(d.gk));
+    bmin = std::min(bmin, d.bits);
+    const int gb = p.C[0] * d.bits;
+    const int cv = gb % 32 == 0 ? (d.gk + gb / 32 - 1) / (gb / 32) : (32 * d.gk + gb - 1) / gb + 1;
+    gcov = std::max(gcov, cv);
+  }
+  const unsigned long long ng = groups ? 2 * p.N / (unsigned long long)p.C[0] : 0;
+  const unsigned long long ystride = ng * (unsigned long long)gks;
   for (int i = p.P - 2; i >= 0; i--) {
     PassArgs b = a;
     if (int rc = pass_args(p, i, b)) return rc;
     b.data = da;
     b.coef = (long long *)db;
     b.fast_round = fast;
+    if (i == 0 && groups) {
+      b.coef = nullptr;
+      b.ybuf = (uint32_t *)db;
+      b.ygK = gks;
+      b.ystride = ystride;
+    }
     const int ci = inv_cols(p, i);
     b.ntiles = (int)(p.N / ((unsigned long long)p.L[i] * (unsigned long long)ci));
     fwd_fn f = pick_inv(p.L[i], ci);
@@ -1309,7 +1333,7 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
     HIPCHK(hipGetLastError());
   }
   // carries, one product per grid row
-  const unsigned long long wpb = (unsigned long long)CT * CWPT;
+  const unsigned long long wpb = groups ? (unsigned long long)GCT * GWPT : (unsigned long long)CT * CWPT;
   unsigned long long nblk_max = 1;
   for (const MulDesc &d : hdesc) {
     unsigned long long nb = (d.cl + wpb - 1) / wpb;
@@ -1334,7 +1358,28 @@ int run_multipass_class(const Plan &p, const MulDesc *ddesc, const std::vector<M
     }
     c.lds_slots = carry_slots(bmin, cbmax);
   }
-  {
+  if (groups) {
+    c.coef = nullptr;
+    c.ybuf = (const uint32_t *)db;
+    c.ystride = ystride;
+    c.gC = p.C[0];
+    c.ng = (long long)ng;
+    c.gK = gks;  // per product from its descriptor
+    c.gKs = gks;
+    // LDS: the groups under one block's GCT * GWPT words at the smallest digit size
+    size_t glds = ((size_t)(32 * GCT * GWPT) / (size_t)(p.C[0] * bmin) + CARRY_GCOV + 2) * (size_t)gks;
+    glds = (glds + glds / 32 + 4) * 4;  // one pad word per 32
+    ProfScope ps(BCG_K_CARRY, s);
+    const dim3 grid((unsigned)nblk_max, (unsigned)count);
+    if (gcov <= 2)
+      hipLaunchKernelGGL(k_carry_groups<2>, grid, dim3(GCT), glds, s, c);
+    else if (gcov == 3)
+      hipLaunchKernelGGL(k_carry_groups<3>, grid, dim3(GCT), glds, s, c);
+    else
+      hipLaunchKernelGGL(k_carry_groups<CARRY_GCOV>, grid, dim3(GCT), glds, s, c);
+    LAUNCH(k_carry_fix, count, 1024, s, c);
+    LAUNCH2(k_carry_apply, (unsigned)nblk_max, (unsigned)count, 256, s, c);
+  } else {
     ProfScope ps(BCG_K_CARRY, s);
     hipLaunchKernelGGL(k_carry_local, dim3((unsigned)nblk_max, (unsigned)count), dim3(CT),
                        c.lds_slots * sizeof(long long), s, c);
@@ -2076,6 +2121,7 @@ int batch_plan(BatchPlan &bp, uint32_t *C, const uint32_t *A, const uint32_t *B,
     d.dl = bp.dl[i];
     d.bits = pl ? pl->bits : 0;
     d.cbits = pl ? coef_bits(*pl) : 0;
+    d.gk = (pl && pl->P >= 2) ? group_words_for(pl->C[0], d.bits, d.cbits) : 0;
     return d;
   };
   std::map<int, std::vector<size_t>> classes;

@@ -15,6 +15,7 @@ struct MulDesc {
   unsigned long long al, bl, cl, dl;
   int bits;
   int cbits;
+  int gk = 0;  // batched group mode: words per group sum of this product's digits (0: none)
 };
 
 struct PassArgs {
@@ -43,6 +44,7 @@ struct PassArgs {
   int ygK;                  // words per group sum (a multiple of 4: 16-byte stores)
   int pair;                 // forward passes of A and B in one launch: grid.y = 1 runs B (data_b, wb, bl)
   int pcount;               // batched forward passes of both operands: grid.y = 2 pcount, rows >= pcount run B
+  unsigned long long ystride;  // batched group mode: group-sum words per product in ybuf
 };
 
 // Batched launches (desc != nullptr): product p = blockIdx.y owns the arrays
@@ -75,6 +77,7 @@ __device__ __forceinline__ void bind_product(PassArgs &a) {
   if (a.data) a.data += p * a.N;
   if (a.data_b) a.data_b += p * a.N;
   if (a.coef) a.coef += p * 2 * a.N;
+  if (a.ybuf) a.ybuf += p * a.ystride;
   if (a.margin) a.margin += p;
 }
 
@@ -893,7 +896,8 @@ struct CarryArgs {
   const uint32_t *ybuf;      // group mode: group sums instead of coefficients (ncoef digits in all)
   int gC, gK, gKs;           // digits per group, words per group sum, group stride (words)
   long long ng;              // groups: ncoef / gC
-  unsigned wpb;              // words per carry block (k_carry_fix)
+  unsigned wpb;              // words per carry block
+  unsigned long long ystride;  // batched group mode: group-sum words per product
 };
 
 __device__ __forceinline__ long long coef_at(const CarryArgs &a, long long j) {
@@ -940,7 +944,11 @@ __device__ __forceinline__ void word_partials(const CarryArgs &a, long long wfir
 __device__ __forceinline__ void bind_carry(CarryArgs &a, unsigned long long p) {
   if (!a.desc) return;
   const MulDesc &d = a.desc[p];
-  a.coef += p * a.coef_stride;
+  if (a.coef) a.coef += p * a.coef_stride;
+  if (a.ybuf) {
+    a.ybuf += p * a.ystride;
+    a.gK = d.gk;
+  }
   a.bits = d.bits;
   a.cbits = d.cbits;
   a.out = d.c;
@@ -948,7 +956,7 @@ __device__ __forceinline__ void bind_carry(CarryArgs &a, unsigned long long p) {
   a.add = d.d;
   a.addl = d.dl;
   a.blk_fn += p * a.nblk;
-  a.nblk = (d.cl + (unsigned long long)CT * CWPT - 1) / ((unsigned long long)CT * CWPT);
+  a.nblk = (d.cl + a.wpb - 1) / a.wpb;
 }
 
 // Coefficient window of one carry block in LDS: words [W0, W0 + CT*CWPT) need
@@ -1125,6 +1133,8 @@ template <int GCOV>  // group sums that can cover one word (<= CARRY_GCOV)
 static __global__ __launch_bounds__(GCT) void k_carry_groups(CarryArgs a) {
   extern __shared__ uint32_t gl_w[];
   __shared__ unsigned wave_fn[GCT / 64];
+  bind_carry(a, blockIdx.y);  // batched: product blockIdx.y
+  if ((unsigned long long)blockIdx.x >= a.nblk) return;
   const int tid = threadIdx.x;
   const long long W0 = (long long)blockIdx.x * GCT * GWPT;
   const int gb = a.gC * a.bits;  // bits per group, >= 32

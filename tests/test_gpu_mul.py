@@ -297,6 +297,25 @@ def test_chunked_groups_at_2p24_words(bc):
     assert np.array_equal(c, c0)
 
 
+@pytest.mark.parametrize("extra", [3, (1 << 19) + 5])
+def test_one_pass_long_outputs(bc, extra):
+    # one-pass products whose output runs past the product: zero words, or a
+    # long addend; from 2^19 output words the workgroup carry takes its
+    # 64-bit form
+    a, b = operands(11, 3000, 2000)
+    cl = a.size + b.size + extra
+    c, info = bc.mul(a, b, cl=cl)
+    assert info["passes"] == 1, info
+    want = exact_product(a, b)
+    assert np.array_equal(c[:want.size], want) and not c[want.size:].any()
+    d = np.random.default_rng(5).integers(0, 1 << 32, cl, dtype=np.uint64).astype(np.uint32)
+    d[-9:] = 0xffffffff  # a carry through the addend's top words
+    c2, info2 = bc.muladd(a, b, d, cl=cl)
+    assert info2["passes"] == 1, info2
+    x = (words_to_int(want) + words_to_int(d)) % (1 << (32 * cl))
+    assert words_to_int(c2) == x
+
+
 def test_autotune_keeps_exactness(bc):
     chosen, times = bc.autotune(300000, 300000, reps=1)
     assert 0 <= chosen < len(times) and all(t > 0 for t in times)

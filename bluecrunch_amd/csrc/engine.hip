@@ -1044,6 +1044,12 @@ int run_transforms(const Plan &p, const uint32_t *A, unsigned long long AL, cons
     b.data = da;
     b.wa = A;
     b.al = AL;
+    {
+      // the packing pass of operands that leave the upper digit half empty
+      // (at most N b / 32 words each) runs the kernel without upper digits
+      const unsigned long long half_w = (p.N * (unsigned long long)p.bits) >> 5;
+      b.lower = (i == 0 && AL <= half_w && (square || BL <= half_w)) ? 1 : 0;
+    }
     if (!square) {
       b.pair = 1;
       b.data_b = db;

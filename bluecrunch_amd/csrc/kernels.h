@@ -44,6 +44,7 @@ struct PassArgs {
   int ygK;                  // words per group sum (a multiple of 4: 16-byte stores)
   int pair;                 // forward passes of A and B in one launch: grid.y = 1 runs B (data_b, wb, bl)
   int pcount;               // batched forward passes of both operands: grid.y = 2 pcount, rows >= pcount run B
+  int lower;                // packing pass of single products: every operand leaves the upper half empty
   unsigned long long ystride;  // batched group mode: group-sum words per product in ybuf
 };
 
@@ -362,6 +363,24 @@ struct PackDigits {
     one = (bits & (bits - 1u)) == 0u;  // b <= 32 divides 32
   }
 
+  // Lower-half digits only (launches whose operands all leave the upper half
+  // empty).
+  __device__ __forceinline__ void load_lower(unsigned (&d)[16]) const {
+    unsigned lo[16];
+#pragma unroll
+    for (int h = 0; h < 16; h++) lo[h] = __builtin_amdgcn_raw_buffer_load_b32(w0, off + (unsigned)h * step, 0, 0);
+    if (one) {
+#pragma unroll
+      for (int h = 0; h < 16; h++) d[h] = __builtin_amdgcn_ubfe(lo[h], sh, bits);
+    } else {
+      unsigned hi[16];
+#pragma unroll
+      for (int h = 0; h < 16; h++) hi[h] = __builtin_amdgcn_raw_buffer_load_b32(w1, off + (unsigned)h * step, 0, 0);
+#pragma unroll
+      for (int h = 0; h < 16; h++) d[h] = __builtin_amdgcn_ubfe(__builtin_amdgcn_alignbit(hi[h], lo[h], sh), 0, bits);
+    }
+  }
+
   // Lower-half digits into d, upper-half digits into e (0 when empty); every
   // load is issued before the first is used.
   __device__ __forceinline__ void load(unsigned (&d)[16], unsigned (&e)[16]) const {
@@ -414,7 +433,9 @@ __device__ __forceinline__ void store_rows16(cd *base, const cd (&v)[16], int r0
   for (int t = 0; t < 16; t++) st_pol<PLAIN>(lb + (unsigned long long)(S * t) * M, v[t]);
 }
 
-template <int L, int C, bool PACK, int CACHE = 0>
+// LOWER (packing pass): every operand of the launch leaves the upper digit
+// half empty, so a_{j+N} = 0 and the weighted point is a_j w exactly.
+template <int L, int C, bool PACK, int CACHE = 0, bool LOWER = false>
 __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   using TC = Tile<L, C>;
   using ST = SplitTile<L, C>;
@@ -450,7 +471,10 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
   if constexpr (PACK) {
     PackDigits pd;
     pd.template init<L>(a, r0, m);
-    pd.load(dre, dim_);
+    if constexpr (LOWER)
+      pd.load_lower(dre);
+    else
+      pd.load(dre, dim_);
     wpack = omega_direct(a.roots, (unsigned long long)r0 * a.M + m);
   }
   if constexpr (SPLIT) {
@@ -458,7 +482,11 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
     cd x[16], y[16];
 #pragma unroll
     for (int t = 0; t < 16; t++) {
-      if constexpr (PACK) {
+      if constexpr (PACK && LOWER) {
+        const cd w = cmul(wpack, w64(t));
+        const double r = (double)dre[t];
+        x[t] = make_double2(r * w.x, r * w.y);
+      } else if constexpr (PACK) {
         // (a_j + i a_{j+N}) w (a_{j+N} = 0 throughout when the upper half is empty)
         x[t] = cmul(make_double2((double)dre[t], (double)dim_[t]), cmul(wpack, w64(t)));
       } else {
@@ -485,7 +513,11 @@ __global__ __launch_bounds__((Tile<L, C>::T)) void k_fwd(PassArgs a) {
 #pragma unroll
     for (int t = 0; t < R0; t++) {
       unsigned long long row = (unsigned long long)(q + t * SUB0);
-      if constexpr (PACK) {
+      if constexpr (PACK && LOWER) {
+        const int h = i + t * PER0;
+        const cd w = cmul(wpack, w64(h));
+        x[t] = make_double2((double)dre[h] * w.x, (double)dre[h] * w.y);
+      } else if constexpr (PACK) {
         const int h = i + t * PER0;
         x[t] = cmul(make_double2((double)dre[h], (double)dim_[h]), cmul(wpack, w64(h)));
       } else {

@@ -16,7 +16,9 @@ int launch_fwd(bool pack, int grid, int gy, hipStream_t s, const PassArgs &a) {
     if (pack) return -2;
     BCG_LAUNCH2((k_fwd<L, C, false>), grid, gy, (Tile<L, C>::T), s, a);
   } else {
-    if (pack)
+    if (pack && a.lower)
+      BCG_LAUNCH2((k_fwd<L, C, true, 0, true>), grid, gy, (Tile<L, C>::T), s, a);
+    else if (pack)
       BCG_LAUNCH2((k_fwd<L, C, true>), grid, gy, (Tile<L, C>::T), s, a);
     else
       BCG_LAUNCH2((k_fwd<L, C, false>), grid, gy, (Tile<L, C>::T), s, a);

@@ -498,7 +498,7 @@ def plan_digit_bias(extra_bits):
 
 def plan_chunk(nbytes, streams=2):
     """Chunked middle passes of three-pass products: group size in bytes of
-    operand points (0: whole-transform passes) and 1 or 2 streams."""
+    operand points (0: whole-transform passes) and 1 to 4 streams."""
     check(lib().bcg_plan_chunk(int(nbytes), int(streams)), "plan_chunk")
 
 

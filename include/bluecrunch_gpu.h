@@ -165,9 +165,10 @@ int bcg_plan_digit_bias(int extra_bits);
  * middle pass and the second inverse pass run group by group of blocks, each
  * group (`bytes` of operand points) kept in the GPU's memory-side cache from
  * one of its launches to the next; 0 runs each pass over the whole transform.
- * streams 2 alternates groups between the call's stream and a side stream
- * (joined back before the product continues), 1 keeps them on one stream.
- * Defaults: 128 MiB, 2. No counterpart in the reference (a tuning knob). */
+ * streams (1 to 4) deals the groups round-robin over the call's stream and
+ * up to three side streams (joined back before the product continues); 1
+ * keeps them on one stream. Defaults: 128 MiB, 2. No counterpart in the
+ * reference (a tuning knob). */
 int bcg_plan_chunk(size_t bytes, int streams);
 /* Index of the candidate transforms of N points use now (forced, autotuned
  * or default); negative BCG_E* code when N has no split. */

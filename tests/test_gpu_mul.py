@@ -260,7 +260,7 @@ def test_chunked_middle_passes(bc, words, group_bytes, exact):
                 assert np.array_equal(s0, want[1]), (i, cands[i])
             else:
                 assert product_fingerprint_ok(a, b, c0) and product_fingerprint_ok(a, a, s0), (i, cands[i])
-            for gb, streams in [(g, k) for g in group_bytes for k in (1, 2)]:
+            for gb, streams in [(g, k) for g in group_bytes for k in (1, 2, 3)]:
                 bc.plan_chunk(gb, streams)
                 c, info = bc.mul(a, b)
                 s, sinfo = bc.mul(a, a)

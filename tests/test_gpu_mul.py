@@ -199,6 +199,34 @@ def test_forced_digit_sizes_ragged(bc, bits):
     assert np.array_equal(dC.to_host(), exact_product(a, b, 40004))
 
 
+@pytest.mark.parametrize("words", [20000, 100000, 300000, 1 << 22])
+@pytest.mark.parametrize("side", ["lower", "upper", "square"])
+def test_packing_lower_half_boundary(bc, words, side):
+    # The packing pass skips the upper digit half (digits N .. 2N - 1) when
+    # both operands fit in h = N b / 32 words. Products on each side of h for
+    # the plan of (words, words): (h, h - 9) and the square of h words skip
+    # it, (h + 1, h - 9) does not. The plan must stay the same so the case
+    # lands on the intended side. The three-pass size is checked by fingerprint.
+    p = bc.plan(words, words)
+    h = p["points"] * p["bits"] // 32
+    al, bl = {"lower": (h, h - 9), "upper": (h + 1, h - 9), "square": (h, h)}[side]
+    assert bc.plan(al, bl) == p
+    a, b = operands(7 * words + len(side), al, bl)
+    if side == "square":
+        b = a
+    cl = al + bl
+    dA, dC = bc.DeviceWords.from_host(a), bc.DeviceWords(cl)
+    dB = dA if side == "square" else bc.DeviceWords.from_host(b)
+    info = bc.mul_dev(dC.ptr, cl, dA.ptr, al, dB.ptr, bl)
+    assert (info["points"], info["bits"], info["passes"]) == (p["points"], p["bits"], p["passes"])
+    assert info["margin"] < 0.25
+    c = dC.to_host()
+    if p["passes"] == 3:
+        assert product_fingerprint_ok(a, b, c)
+    else:
+        assert np.array_equal(c, exact_product(a, b))
+
+
 @pytest.mark.parametrize("words", [20011, 300007])
 def test_muladd_group_carry(bc, words):
     # a*b + d through the group-sum carry (multi-pass sizes): the addend's

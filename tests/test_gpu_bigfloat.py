@@ -54,6 +54,29 @@ def test_add_sub_long_carries_and_large(bc, oracle):
                 assert same(dev(bc, name, a, b, p), want), (name, p)
 
 
+def test_add_sub_of_nearly_equal_magnitudes(bc, oracle):
+    # |a| and |b| agree on every word above one position: deep in a 10^5-word
+    # window, at its top, nowhere, or only in b's extra low word (b one word
+    # longer, one exponent lower). The magnitude comparison of a same-sign sub
+    # and an opposite-sign add scans down to that position.
+    rng = np.random.default_rng(14)
+    n = 100000
+    w = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    w[-1] |= np.uint32(1)
+    cases = []
+    for k in (0, 3, 300, 70000, n - 1, None):
+        v = w.copy()
+        if k is not None:
+            v[k] ^= np.uint32(2)  # the top word keeps its bit 0
+        cases.append(((w, 3, 1), (v, 3, 1)))
+    cases.append(((w, 3, 1), (np.concatenate([np.array([5], np.uint32), w]), 2, 1)))
+    for a, b in cases:
+        for name, bb in (("sub", b), ("add", (b[0], b[1], 1 - b[2]))):
+            for p in (0, 1000):
+                want = getattr(oracle, "bigfloat_" + name)(a, bb, p)
+                assert same(dev(bc, name, a, bb, p), want), (name, p)
+
+
 def test_in_place_targets(bc, oracle):
     rng = np.random.default_rng(13)
     for i in range(100):

@@ -398,7 +398,8 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the FFT multiply runs on a single MI355X "
-              "(multi-GPU sharding is a design note, not part of this build).", file=sys.stderr)
+              "(the library drives one device per process; sharding a product over GPUs is not implemented).",
+              file=sys.stderr)
         sys.exit(2)
 
     import bluecrunch_amd as bc

@@ -5,7 +5,7 @@ import bluecrunch_amd as bc
 
 
 def test_e_terms_known_sizes():
-    # the survey's restated sizes for 1e8 and 1e9 digits
+    # series lengths for 1e8 and 1e9 digits (p = digits / 9 + 1 words)
     assert bc.e_terms(11111112) == 15821668
     assert bc.e_terms(111111112) == 138854691
 

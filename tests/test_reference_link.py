@@ -1,4 +1,5 @@
-"""The drop-in recipe of INTEGRATION.md, checked on the reference's own callers:
+"""The drop-in recipe (the reference builds without its multiply sources and
+links libbluecrunch.so), checked on the reference's own callers:
 every C file the recipe keeps (main.c, bench.c, bigfloat/*.c but mul.c,
 util/bench.c, fft/fftnew.c and the test programs) compiles with the reference's
 warning flags and its own headers first, and every program links against
@@ -9,7 +10,7 @@ Compile flags: the reference Makefile's CFLAGS (Makefile:8) without the three
 parts this image cannot honour: -fcilkplus (no Cilk Plus runtime: the
 keywords come from the serial-elision stand-in in tests/refbuild/cilk),
 -fprofile-use=. (no profile data) and -flto (irrelevant to linking). Skipped
-where the reference tree is not present (the GPU box)."""
+where the reference tree is not present."""
 import os
 import shutil
 import subprocess
@@ -22,7 +23,7 @@ import bluecrunch_amd as bc
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = "/root/reference"
 CFLAGS = ["-std=c11", "-Wall", "-Werror", "-O3", "-no-pie", "-mavx", "-funroll-loops"]
-# the files INTEGRATION.md drops from the reference build (replaced by the
+# the files the recipe drops from the reference build (replaced by the
 # library): the multiply path, and optionally also sum, reciprocal and quotient
 DROPPED = {"fft/fft.c", "fft/baileys.c", "fft/tft.c", "fft/pointwise.c", "fft/convert.c", "fft/table.c",
            "bigfloat/mul.c"}

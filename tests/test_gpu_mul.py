@@ -22,6 +22,23 @@ def test_mul_exact(bc, al, bl):
     assert info["margin"] < 0.25
 
 
+def test_mul_random_shapes(bc):
+    # 80 seeded operand shapes, lengths log-uniform in [1, 3e5] words (every
+    # planner digit size, 2^k and 3 2^k lengths, one- and two-pass plans,
+    # balanced and lopsided operands, either side of the lower-half bound),
+    # each product exact with its margin below 0.25
+    rng = np.random.default_rng(2024)
+    plans = set()
+    for i in range(80):
+        al, bl = (int(np.exp(rng.uniform(0, np.log(3e5)))) for _ in range(2))
+        a, b = operands(50000 + i, al, bl)
+        c, info = bc.mul(a, b)
+        assert np.array_equal(c, exact_product(a, b)), (al, bl, info)
+        assert info["margin"] < 0.25, (al, bl, info)
+        plans.add((info["points"], info["bits"], info["passes"]))
+    assert len(plans) > 30
+
+
 @pytest.mark.parametrize("al", [1, 60, 100, 1024, 5000, 65536, 262144])
 def test_square_exact(bc, al):
     a, _ = operands(al, al, 1)

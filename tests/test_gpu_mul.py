@@ -39,6 +39,30 @@ def test_mul_random_shapes(bc):
     assert len(plans) > 30
 
 
+def test_random_shapes_muladd_square_alias(bc):
+    # 40 seeded shapes up to 1e5 words: a*b + d truncated to CL words (CL
+    # below, at and above AL + BL; addends shorter and longer than the
+    # product), the square of a, and the product written over A in place
+    import ctypes
+    rng = np.random.default_rng(77)
+    for i in range(40):
+        al, bl = (int(np.exp(rng.uniform(0, np.log(1e5)))) for _ in range(2))
+        a, b = operands(60000 + i, al, bl)
+        dl = int(rng.integers(0, al + bl + 4))
+        d = operands(70000 + i, max(dl, 1), 1)[0][:dl]
+        cl = max(1, al + bl + int(rng.integers(-3, 3)))
+        c, info = bc.muladd(a, b, d, cl)
+        want = (words_to_int(a) * words_to_int(b) + words_to_int(d)) % (1 << (32 * cl))
+        assert words_to_int(c) == want, ("muladd", al, bl, dl, cl, info)
+        s, info = bc.mul(a, a)
+        assert np.array_equal(s, exact_product(a, a)), ("square", al, info)
+        dA = bc.DeviceWords(al + bl)
+        bc.check(bc.lib().bcg_memcpy_h2d(dA.ptr, a.ctypes.data, 4 * al))
+        dB = bc.DeviceWords.from_host(b)
+        info = bc.mul_dev(dA.ptr, al + bl, dA.ptr, al, dB.ptr, bl)
+        assert np.array_equal(dA.to_host(), exact_product(a, b)), ("alias", al, bl, info)
+
+
 @pytest.mark.parametrize("al", [1, 60, 100, 1024, 5000, 65536, 262144])
 def test_square_exact(bc, al):
     a, _ = operands(al, al, 1)

@@ -80,6 +80,27 @@ def test_muladd_batch_binary_splitting_shape(bc):
         assert words_to_int(c) == want
 
 
+def test_muladd_batch_random(bc):
+    # 300 seeded products, lengths log-uniform in [1, 40000] words, addends of
+    # 0 to CL + 2 words, outputs from CL = AL + BL - 2 to AL + BL + 2 words: tiny,
+    # one-pass and multi-pass classes in one call, each product exact
+    from helpers import words_to_int
+    rng = np.random.default_rng(31)
+    sm = SplitMix64(31)
+    pairs, adds, cls = [], [], []
+    for _ in range(300):
+        al, bl = (int(np.exp(rng.uniform(0, np.log(40000)))) for _ in range(2))
+        cl = max(1, al + bl + int(rng.integers(-2, 3)))
+        pairs.append((operand(sm, al), operand(sm, bl)))
+        adds.append(operand(sm, int(rng.integers(0, cl + 3))))
+        cls.append(cl)
+    out, worst = bc.mul_batch(pairs, cls=cls, adds=adds)
+    for i, ((a, b), d, c, cl) in enumerate(zip(pairs, adds, out, cls)):
+        want = (words_to_int(a) * words_to_int(b) + words_to_int(d)) % (1 << (32 * cl))
+        assert words_to_int(c) == want, (i, len(a), len(b), len(d), cl)
+    assert worst < 0.25
+
+
 def _device_batch(bc, pairs, adds=None):
     al = np.array([len(a) for a, _ in pairs], dtype=np.uint64)
     bl = np.array([len(b) for _, b in pairs], dtype=np.uint64)

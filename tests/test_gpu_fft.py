@@ -9,7 +9,8 @@ from helpers import GlibcRand, brute_dft, reference_order, words_to_int
 pytestmark = pytest.mark.gpu
 
 LENGTHS = [2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512, 768, 1024, 2048, 3072, 4096, 6144, 8192,
-           12288, 16384, 1 << 15, 3 << 14, 1 << 17, 3 << 16, 1 << 20, 3 << 19]
+           12288, 16384, 1 << 15, 3 << 14, 1 << 17, 3 << 16, 1 << 18, 3 << 17, 1 << 19, 3 << 18, 1 << 20, 3 << 19,
+           1 << 21, 3 << 20, 1 << 22]
 
 
 def cplx(pairs):

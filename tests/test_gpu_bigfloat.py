@@ -111,6 +111,16 @@ def test_rcp_matches_oracle(bc, oracle, p):
         assert same(bc.bigfloat_rcp(a, p), want), (p, k)
 
 
+@pytest.mark.parametrize("p", [20000, 120000, 400000])
+def test_rcp_large_matches_oracle(bc, oracle, p):
+    # precisions whose Newton products run as multi-pass transforms
+    rng = np.random.default_rng(17 + p)
+    for k in range(2):
+        a = rand_bf(rng, p, p + 30, 10, zero_ok=False) if k else rand_bf(rng, p // 3, p // 2, 10, zero_ok=False)
+        want = oracle.bigfloat_rcp(a, p)
+        assert same(dev(bc, "rcp", a, None, p), want), (p, k)
+
+
 @pytest.mark.parametrize("words", [50, 700, 5000, 60000])
 def test_div_matches_oracle(bc, oracle, words):
     rng = np.random.default_rng(16 + words)

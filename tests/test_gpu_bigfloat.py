@@ -121,7 +121,7 @@ def test_rcp_large_matches_oracle(bc, oracle, p):
         assert same(dev(bc, "rcp", a, None, p), want), (p, k)
 
 
-@pytest.mark.parametrize("words", [50, 700, 5000, 60000])
+@pytest.mark.parametrize("words", [50, 700, 5000, 60000, 250000])
 def test_div_matches_oracle(bc, oracle, words):
     rng = np.random.default_rng(16 + words)
     for _ in range(3):

@@ -88,6 +88,15 @@ def test_e_digits_md5(bc, golden, D):
         assert txt == "2." + e_digits_exact(D)[1:]
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("D", [31337, 123457])
+def test_e_digits_irregular_counts_exact(bc, D):
+    # digit counts that are not powers of ten (a partial last group of nine
+    # digits, odd tree shapes) against floor(e 10^D) from integer arithmetic
+    txt, ms = bc.e_digits(D)
+    assert txt == "2." + e_digits_exact(D)[1:]
+
+
 def test_e_digits_1e6_md5(bc, golden):
     import hashlib
     txt, ms = bc.e_digits(1000000)
